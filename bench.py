@@ -1,0 +1,191 @@
+"""Benchmark of the pil2-stark commitment hot path on one MI355X.
+
+Default workload (the plain-large configuration): one trace of 2^22 rows x 128
+uniform-random Goldilocks columns in HBM (row-major, as the CPU path hands it
+over), LDE at blowup 4 to 2^24 rows, then the Poseidon2 (width 12, arity 3)
+Merkle commit over the LDE rows. One step = one full commit (LDE + tree + root).
+
+value = commit GElem/s = NExt * nCols / seconds per commit, inputs resident in
+HBM when the timed region starts. The dominant kernel (Poseidon2 leaf hashing)
+and the LDE passes are timed with HIP events on the commit's own stream.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain_large]
+       [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "pil2-proofman_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "commit GElem/s (LDE+Poseidon2 Merkle), batched vs plain; p50/p99 latency"
+
+CONFIGS = {
+    "plain_large": dict(n_bits=22, blowup_bits=2, cols=128, arity=3, seed=0x5EED0002),
+    "plain_small": dict(n_bits=16, blowup_bits=1, cols=32, arity=3, seed=0x5EED0001),
+}
+
+# MI355X published peaks
+HBM_PEAK_GBS = 8000.0
+# VALU lane-operations per second: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+# (= the published 157.3 TFLOP/s FP32 vector rate / 2 for FMA / 2 for packed).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Poseidon2 field multiplications per permutation (algorithmic count):
+# 8 full rounds x W x 4 (x^7) + 22 partial rounds x (4 + W)
+MULS_PER_PERM = {8: 8 * 8 * 4 + 22 * 12, 12: 8 * 12 * 4 + 22 * 16, 16: 8 * 16 * 4 + 22 * 20}
+
+
+def host_info():
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(cfg, budget_cols=2):
+    """Time the reference's own CPU path (oracle/_ref: NTT_Goldilocks::LDE +
+    Poseidon2Goldilocks<W>::merkletree, OpenMP over all host cores) on a bounded
+    sample of the same workload: full trace height, same blowup and arity,
+    `budget_cols` columns. Falls back to our C restatement if the reference
+    library cannot be loaded here."""
+    import numpy as np
+    import datagen
+    from oracle.oracle import Oracle, RefLib
+    n, ne, arity = 1 << cfg["n_bits"], 1 << (cfg["n_bits"] + cfg["blowup_bits"]), cfg["arity"]
+    cols = budget_cols
+    trace = datagen.uniform(n * cols, cfg["seed"]).reshape(n, cols)
+    o = Oracle()
+    try:
+        lib, kind = RefLib(), "reference"
+        tree_len = o.tree_num_elements(ne, arity)
+        run = lambda: lib.merkletree(lib.lde(trace, ne), arity, tree_len)
+        threads = lib.num_threads()
+    except (OSError, FileNotFoundError):
+        lib, kind = o, "port"
+        run = lambda: o.merkletree(o.lde(trace, ne), arity)
+        threads = o.num_threads()
+    t0 = time.perf_counter()
+    run()
+    dt = time.perf_counter() - t0
+    return {"value": ne * cols / dt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+            "sample": "2^%d x %d cols -> 2^%d rows (blowup %d), Poseidon2 W%d arity %d; %.2f s"
+                      % (cfg["n_bits"], cols, cfg["n_bits"] + cfg["blowup_bits"], 1 << cfg["blowup_bits"],
+                         4 * arity, arity, dt),
+            **host_info()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="plain_large", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cols", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the commit path shards per job / per column group, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+    import torch
+    import pil2_commit as pc
+
+    cfg = CONFIGS[args.config]
+    nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
+    n, ne, width = 1 << nb, 1 << nbe, 4 * arity
+    pc.set_hash_family(2)
+    dev = torch.device("cuda:0")
+    trace = pc.empty(n * cols, dev)
+    pc.fill_uniform(trace, cfg["seed"])
+    lde = pc.empty(ne * cols, dev)
+    nodes = pc.empty(pc.get_tree_num_elements(ne, arity), dev)
+    scratch = pc.empty(n * cols, dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
+    torch.cuda.synchronize()
+
+    # timed region: K commits; phase events on the commit stream
+    pc.lib.pil2hip_timing_read(None, True)
+    pc.lib.pil2hip_timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pc.lib.pil2hip_timing_enable(False)
+    ph = (ctypes.c_double * 6)()
+    ncalls = pc.lib.pil2hip_timing_read(ph, True)
+    phase_ms = [ph[i] / max(ncalls, 1) for i in range(6)]
+    root = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
+
+    ms_per_step = dt * 1e3 / args.steps
+    value = ne * cols / (dt / args.steps) / 1e9
+
+    # dominant kernel: Poseidon2 leaf hashing (k_p2_leaves<W>)
+    rate = width - 4
+    perms = ne * ((cols + rate - 1) // rate)
+    leaf_s = phase_ms[4] / 1e3
+    leaf_bytes = 8 * ne * cols + 32 * ne  # reads every LDE word once, writes one digest per row
+    muls_per_s = perms * MULS_PER_PERM[width] / leaf_s
+    roofline = {
+        "bound": "valu", "kernel": "k_p2_leaves<%d>" % width,
+        "achieved": round(perms / leaf_s / 1e9, 4), "unit": "Gperm/s",
+        "peak": None, "frac": None,
+        "field_muls_per_s": muls_per_s,
+        "valu_peak_lane_ops_per_s": VALU_PEAK_TOPS * 1e12,
+        "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
+        "traffic": None,
+        "avg_ms": phase_ms[4],
+    }
+    # LDE passes: algorithmic bytes 8*C*(N + NExt) over the four LDE kernels
+    lde_ms = sum(phase_ms[0:4])
+    lde_bytes = 8 * cols * (n + ne)
+    pass_bytes = 16 * cols * n * 2 + 8 * cols * (n + ne) + 16 * cols * ne  # each pass reads+writes its array
+    roofline_lde = {
+        "bound": "hbm", "kernels": "transpose + strided iNTT + fused block + strided NTT",
+        "achieved": lde_bytes / (lde_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": lde_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "per_pass_achieved": pass_bytes / (lde_ms / 1e3) / 1e9,
+        "per_pass_frac": pass_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": None, "avg_ms": lde_ms,
+    }
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks field, exact integer)",
+        "data": "synthetic: seeded uniform Goldilocks elements generated in HBM",
+        "config": {"workload": "%s: trace 2^%d x %d -> LDE 2^%d (blowup %d), Poseidon2 W%d arity %d Merkle commit"
+                               % (args.config, nb, cols, nbe, 1 << cfg["blowup_bits"], width, arity),
+                   "n_bits": nb, "n_bits_ext": nbe, "n_cols": cols, "arity": arity, "seed": cfg["seed"],
+                   "path": "plain (one instance per call)"},
+        "phases_ms": dict(zip(["transpose", "intt_strided", "lde_block", "ntt_strided", "leaves", "tree_levels"],
+                              [round(x, 4) for x in phase_ms])),
+        "roofline": roofline, "roofline_lde": roofline_lde,
+        "root": ["%016x" % v for v in root],
+        "device": torch.cuda.get_device_name(0), "host": platform.node(),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_cols)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

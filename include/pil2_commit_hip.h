@@ -1,0 +1,137 @@
+/*
+ * pil2_commit_hip.h — C-ABI of libpil2_commit_hip.so, the MI355X implementation of
+ * pil2-stark's polynomial-commitment path (Goldilocks LDE, Poseidon2 Merkle commit,
+ * FRI fold).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-ins for the reference ABI (pil2-stark/src/api/starks_api.hpp). Same
+ *     names, argument order and meaning; proofman's Rust FFI
+ *     (provers/starks-lib-c/src/ffi_starks.rs) links them unchanged. Buffers are
+ *     host memory, row-major, exactly as the CPU backend takes them.
+ *
+ *  2. Device-resident entry points (prefix pil2hip_) with no single reference
+ *     counterpart: the same operations on buffers already in HBM, launched on a
+ *     caller-supplied hipStream_t passed as void* (NULL = the null stream). They
+ *     return 0 on success and a negative code on error; pil2hip_last_error()
+ *     then gives a message. Device layouts:
+ *       trace  row-major  N x nCols   (element (r, c) at r*nCols + c)
+ *       LDE    col-major  nCols x NExt (element (r, c) at c*NExt + r)
+ *       nodes  the reference Merkle layout (goldilocks_tooling.hpp:9-21):
+ *              leaf digests (4 words per row), zero pad digests, next level,
+ *              ..., root = last 4 words; pil2hip_tree_num_elements() words.
+ *
+ * All field values written by this library are canonical (< p). Inputs may hold
+ * any 64-bit word; they are reduced mod p on load, matching the reference, whose
+ * results are compared through Goldilocks::toU64.
+ */
+#ifndef PIL2_COMMIT_HIP_H
+#define PIL2_COMMIT_HIP_H
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ reference ABI drop-ins -- */
+
+/* starks_api.hpp:28 (impl starks_api.cpp:43). 1 = Poseidon1, 2 = Poseidon2.
+ * Only Poseidon2 is implemented; commits under family 1 (or unset) fail. */
+void set_hash_family(uint8_t fam);
+
+/* starks_api.hpp:96 (CPU impl starks_api.cpp:633-681). buffer: host row-major
+ * trace of 2^nBits x nCols u64; root: 4 u64 out; bufferFile: "" for none, else
+ * the file [root 32 B | trace | LDE (row-major) | tree nodes] is written, as the
+ * CPU backend writes it. arity 2/3/4 selects Poseidon2 width 8/12/16. d_buffers
+ * may be NULL or a handle from gen_device_buffers. Errors print a message and
+ * abort, as the reference does (exitProcess), unless
+ * pil2hip_set_abort_on_error(false) was called. */
+void write_custom_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                         void *d_buffers, void *buffer, char *bufferFile);
+
+/* starks_api.hpp:190 / :193. Returns an opaque handle that owns a HIP stream on
+ * the current device (default priority). node_rank/node_size/numa_nodes are
+ * accepted for signature compatibility; this build uses one GPU per process. */
+void *gen_device_buffers(uint32_t node_rank, uint32_t node_size, const int32_t *numa_nodes, uint32_t arity,
+                         uint32_t max_n_bits_ext);
+void free_device_buffers(void *d_buffers);
+
+/* starks_api.hpp:203 */
+uint64_t get_num_gpus(void);
+
+/* starks_api.hpp:230 (starks_backend.cpp:233-243). This library is the GPU
+ * backend only: set_gpu_mode(true) returns true, set_gpu_mode(false) returns
+ * false (there is no CPU fallback inside this library). */
+bool set_gpu_mode(bool use_gpu);
+
+/* ---------------------------------------------------- device-resident path -- */
+
+/* getTreeNumElements (goldilocks_tooling.hpp:9-21): u64 words in the tree buffer. */
+uint64_t pil2hip_tree_num_elements(uint64_t nRows, uint64_t arity);
+
+/* NTT_Goldilocks::LDE (ntt_goldilocks.cpp:369-404) on device:
+ * d_lde[c*NExt + j] = p_c(7 * w_NExt^j), p_c interpolating column c of d_trace.
+ * d_scratch: >= 2^nBits * nCols words. nBitsExt - nBits >= 0, nBitsExt <= 24. */
+int64_t pil2hip_lde(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt,
+                    uint64_t nCols, void *stream);
+
+/* Poseidon2Goldilocks<4*arity>::merkletree (poseidon2_goldilocks.cpp:130-175) as
+ * called by MerkleTreeGL::merkelize (merkleTreeGL.cpp:316-336). Row r, column c
+ * of the source is d_src[r*rowStride + c*colStride]. */
+int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRows, uint64_t nCols, uint64_t rowStride,
+                          uint64_t colStride, uint64_t arity, void *stream);
+
+/* LDE followed by merkelize over the LDE rows: commit_witness_cpu's core
+ * (starks_api.cpp:683-723) without SetupCtx. */
+int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
+                       uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
+
+/* MerkleTreeGL::getRoot (merkleTreeGL.cpp:103-107): synchronous copy of the root
+ * (4 words) into host memory. */
+int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, void *stream);
+
+/* FRI::fold (fri.hpp:32-99) for step > 0: d_out[g] (g < 2^currentBits, cubic,
+ * 3 words each) from d_pol (2^prevBits cubic values). challenge: 3 words in host
+ * memory. d_out may equal d_pol (in place, as the reference). step == 0 is a no-op. */
+int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t *challenge, uint64_t step,
+                         uint64_t nBitsExt, uint64_t prevBits, uint64_t currentBits, void *stream);
+
+/* FRI getTransposed (fri.hpp:209-226): d_aux[(i*h + j)] = d_pol[(j*w + i)],
+ * w = 2^tBits, h = degree / w, 3-word elements. */
+int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t degree, uint64_t tBits, void *stream);
+
+/* Column-major -> row-major copy (e.g. to hand an LDE back in the CPU layout). */
+int64_t pil2hip_transpose_cm2rm(uint64_t *d_out, const uint64_t *d_in, uint64_t rows, uint64_t cols,
+                                uint64_t inColStride, uint64_t outRowStride, void *stream);
+
+/* Poseidon2Goldilocks<width>::permute (poseidon2_goldilocks.cpp:10-41) on
+ * `count` consecutive states of `width` words (8, 12 or 16), in place. */
+int64_t pil2hip_p2_permute(uint64_t *d_states, uint64_t count, uint32_t width, void *stream);
+
+/* Synthetic inputs: d_out[i] = element (start + i) of the seeded uniform stream
+ * over [0, p) (counter-based SplitMix64, re-mixed while >= p). Not part of the
+ * reference ABI; used to build benchmark inputs directly in HBM. */
+int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_t start, void *stream);
+
+/* Phase timing of pil2hip_commit with HIP events on the commit's own stream (no
+ * reference counterpart; the reference's TimerGPU, utils/gpu_timer.cuh, plays
+ * this role). While enabled, every pil2hip_commit synchronises its stream and
+ * adds the elapsed milliseconds of its phases to accumulators:
+ * [0] row->column transpose, [1] iNTT strided pass, [2] fused iNTT/coset/NTT
+ * block, [3] NTT strided pass, [4] Poseidon2 leaf hashing, [5] tree levels.
+ * pil2hip_timing_read copies the 6 sums and returns the number of timed commits. */
+void pil2hip_timing_enable(bool on);
+uint64_t pil2hip_timing_read(double *phase_ms, bool reset);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *pil2hip_last_error(void);
+
+/* When true (default, as the reference), errors inside the void reference
+ * entry points print and abort; when false they only record the error. */
+void pil2hip_set_abort_on_error(bool abort_on_error);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,9 @@
+// Launch interface of the FRI kernels (internal to libpil2_commit_hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challenge, uint64_t n_bits_ext,
+                               uint64_t prev_bits, uint64_t cur_bits, hipStream_t st);
+extern "C" int fri_launch_transpose(uint64_t *aux, const uint64_t *pol, uint64_t degree, uint32_t t_bits,
+                                    hipStream_t st);

@@ -1,0 +1,137 @@
+// FRI fold and FRI-layer regrouping kernels for gfx950.
+//
+// k_fri_fold restates FRI::fold (pil2-stark/src/starkpil/fri/fri.hpp:32-99) with
+// polMulAxi (:178-186) and evalPol (:189-206): for each folded index g in
+// [0, 2^cur), gather the nX = 2^(prev-cur) cubic values pol[i*2^cur + g],
+// interpolate them (inverse DFT of size nX), scale coefficient i by sinv_g^i with
+// sinv_g = (7^(2^(nBitsExt-prev)) * w_{2^prev}^g)^-1, and evaluate at the
+// challenge with Horner. One lane per g; the nX-point DFT runs in registers.
+//
+// k_fri_transpose restates getTransposed (fri.hpp:209-226): aux[i*h + j] =
+// pol[j*w + i] for w = 2^nextBits, h = 2^cur / w (elements of 3 words).
+#include "gl_field.h"
+#include "fri_kernels.h"
+
+template <int LOGX>
+__global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, const uint64_t *__restrict__ pol,
+                                                  uint32_t cur_bits, uint64_t shift_inv, uint64_t w_inv,
+                                                  uint64_t ch0, uint64_t ch1, uint64_t ch2, uint64_t wx_inv,
+                                                  uint64_t nx_inv)
+{
+    constexpr int NX = 1 << LOGX;
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n2 = 1ull << cur_bits;
+    if (g >= n2) return;
+    uint64_t a[3][NX];
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+        // bit-reversed placement so the DIT below yields natural order
+        int p = (int)bitrev32((uint32_t)i, LOGX);
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[k][p] = gl_canon(pol[((uint64_t)i * n2 + g) * 3 + k]);
+    }
+    // inverse DFT (radix-2 DIT, roots of w_NX^-1)
+#pragma unroll
+    for (int t = 0; t < LOGX; t++) {
+        int h = 1 << t;
+        uint64_t wstep = wx_inv;
+#pragma unroll
+        for (int q = t + 1; q < LOGX; q++) wstep = gl_mul(wstep, wstep);  // w_{2h}^-1
+        uint64_t w = 1;
+#pragma unroll
+        for (int j = 0; j < h; j++) {
+#pragma unroll
+            for (int b = 0; b < NX; b += 2 * h) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    uint64_t u = a[k][b + j];
+                    uint64_t v = j ? gl_mul(a[k][b + j + h], w) : a[k][b + j + h];
+                    a[k][b + j] = gl_add(u, v);
+                    a[k][b + j + h] = gl_sub(u, v);
+                }
+            }
+            w = gl_mul(w, wstep);
+        }
+    }
+    // sinv_g = shift_inv * w_inv^g ; coefficient i scaled by nx_inv * sinv_g^i
+    uint64_t sinv = gl_mul(shift_inv, gl_pow(w_inv, g));
+    uint64_t r = nx_inv;
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[k][i] = gl_mul(a[k][i], r);
+        r = gl_mul(r, sinv);
+    }
+    gl3 x = {{ch0, ch1, ch2}};
+    gl3 acc = {{a[0][NX - 1], a[1][NX - 1], a[2][NX - 1]}};
+#pragma unroll
+    for (int i = NX - 2; i >= 0; i--) {
+        gl3 c = {{a[0][i], a[1][i], a[2][i]}};
+        acc = gl3_add(gl3_mul(acc, x), c);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[g * 3 + k] = acc.v[k];
+}
+
+__global__ __launch_bounds__(256) void k_fri_transpose(uint64_t *__restrict__ aux, const uint64_t *__restrict__ pol,
+                                                       uint64_t degree, uint32_t t_bits)
+{
+    uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // destination element index i*h + j
+    if (d >= degree) return;
+    uint64_t w = 1ull << t_bits, h = degree >> t_bits;
+    uint64_t i = d / h, j = d - i * h;
+    uint64_t s = j * w + i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) aux[d * 3 + k] = pol[s * 3 + k];
+}
+
+extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challenge, uint64_t n_bits_ext,
+                               uint64_t prev_bits, uint64_t cur_bits, hipStream_t st)
+{
+    if (cur_bits > prev_bits || prev_bits > n_bits_ext || prev_bits > 32) return -1;
+    uint32_t logx = (uint32_t)(prev_bits - cur_bits);
+    uint64_t shift = 7;
+    for (uint64_t j = 0; j < n_bits_ext - prev_bits; j++) shift = gl_mul(shift, shift);
+    uint64_t shift_inv = gl_inv(shift);
+    static const uint64_t W[33] = {
+        0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
+        2198989700608ULL, 4404853092538523347ULL, 6434636298004421797ULL, 4255134452441852017ULL,
+        9113133275150391358ULL, 4355325209153869931ULL, 4308460244895131701ULL,
+        7126024226993609386ULL, 1873558160482552414ULL, 8167150655112846419ULL,
+        5718075921287398682ULL, 3411401055030829696ULL, 8982441859486529725ULL,
+        1971462654193939361ULL, 6553637399136210105ULL, 8124823329697072476ULL,
+        5936499541590631774ULL, 2709866199236980323ULL, 8877499657461974390ULL,
+        3757607247483852735ULL, 4969973714567017225ULL, 2147253751702802259ULL,
+        2530564950562219707ULL, 1905180297017055339ULL, 3524815499551269279ULL,
+        7277203076849721926ULL};
+    uint64_t w_inv = gl_inv(W[prev_bits]);
+    uint64_t wx_inv = gl_inv(W[logx]);
+    uint64_t nx_inv = gl_inv((1ull << logx) % GL_P);
+    uint64_t c0 = gl_canon(challenge[0]), c1 = gl_canon(challenge[1]), c2 = gl_canon(challenge[2]);
+    uint64_t n2 = 1ull << cur_bits;
+    dim3 g((unsigned)((n2 + 255) / 256));
+#define FOLD_CASE(L)                                                                                        \
+    case L:                                                                                                 \
+        hipLaunchKernelGGL(k_fri_fold<L>, g, dim3(256), 0, st, out, pol, (uint32_t)cur_bits, shift_inv,     \
+                           w_inv, c0, c1, c2, wx_inv, nx_inv);                                              \
+        break;
+    switch (logx) {
+        FOLD_CASE(1)
+        FOLD_CASE(2)
+        FOLD_CASE(3)
+        FOLD_CASE(4)
+        FOLD_CASE(5)
+    default: return -2;
+    }
+#undef FOLD_CASE
+    return (int)hipGetLastError();
+}
+
+extern "C" int fri_launch_transpose(uint64_t *aux, const uint64_t *pol, uint64_t degree, uint32_t t_bits,
+                                    hipStream_t st)
+{
+    if (degree == 0) return 0;
+    dim3 g((unsigned)((degree + 255) / 256));
+    hipLaunchKernelGGL(k_fri_transpose, g, dim3(256), 0, st, aux, pol, degree, t_bits);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,11 @@
+// Launch interface of the Poseidon2 Merkle kernels (internal to libpil2_commit_hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define P2_THREADS 256
+
+extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
+                                uint64_t nrows, uint64_t *out, hipStream_t st);
+extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st);
+extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);

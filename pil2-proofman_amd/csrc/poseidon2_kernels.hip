@@ -1,0 +1,233 @@
+// Poseidon2-Goldilocks Merkle commitment kernels for gfx950.
+//
+// Semantics:
+//   permutation   pil2-stark/src/goldilocks/src/poseidon2_goldilocks.cpp:10-41
+//                 (external layer, 4 full rounds, 22 partial rounds, 4 full rounds;
+//                  external layer = M4 on each 4-block + column sums,
+//                  poseidon2_goldilocks.hpp:186-231)
+//   leaf sponge   poseidon2_goldilocks.cpp:43-77 (rate W-4, capacity 4, capacity
+//                 of block k>0 = state[0..4) of block k-1, zero-padded rate,
+//                 empty row -> zero digest)
+//   tree          poseidon2_goldilocks.cpp:130-175 (levels padded with zero digests
+//                 to a multiple of arity, parent = first 4 words of permute(children))
+//   width         W = 4 * arity (merkleTreeGL.cpp:316-336)
+//
+// One permutation per lane: the state lives in VGPRs, round constants are
+// wave-uniform and come from __constant__ memory through scalar loads.
+#include "gl_field.h"
+#include "p2_constants.h"
+#include "poseidon2_kernels.h"
+
+__constant__ uint64_t c_rc8[86] = P2_RC8_INIT;
+__constant__ uint64_t c_rc12[118] = P2_RC12_INIT;
+__constant__ uint64_t c_rc16[150] = P2_RC16_INIT;
+__constant__ uint64_t c_d8[8] = P2_DIAG8_INIT;
+__constant__ uint64_t c_d12[12] = P2_DIAG12_INIT;
+__constant__ uint64_t c_d16[16] = P2_DIAG16_INIT;
+
+template <int W> struct P2Tab;
+template <> struct P2Tab<8> {
+    static __device__ __forceinline__ uint64_t rc(int i) { return c_rc8[i]; }
+    static __device__ __forceinline__ uint64_t d(int i) { return c_d8[i]; }
+};
+template <> struct P2Tab<12> {
+    static __device__ __forceinline__ uint64_t rc(int i) { return c_rc12[i]; }
+    static __device__ __forceinline__ uint64_t d(int i) { return c_d12[i]; }
+};
+template <> struct P2Tab<16> {
+    static __device__ __forceinline__ uint64_t rc(int i) { return c_rc16[i]; }
+    static __device__ __forceinline__ uint64_t d(int i) { return c_d16[i]; }
+};
+
+__device__ __forceinline__ uint64_t dbl(uint64_t a) { return gl_add(a, a); }
+
+// M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] applied with additions only.
+__device__ __forceinline__ void m4(uint64_t &x0, uint64_t &x1, uint64_t &x2, uint64_t &x3)
+{
+    uint64_t t0 = gl_add(x0, x1);
+    uint64_t t1 = gl_add(x2, x3);
+    uint64_t t2 = gl_add(dbl(x1), t1);
+    uint64_t t3 = gl_add(dbl(x3), t0);
+    uint64_t t4 = gl_add(dbl(dbl(t1)), t3);
+    uint64_t t5 = gl_add(dbl(dbl(t0)), t2);
+    uint64_t t6 = gl_add(t3, t5);
+    uint64_t t7 = gl_add(t2, t4);
+    x0 = t6; x1 = t5; x2 = t7; x3 = t4;
+}
+
+template <int W>
+__device__ __forceinline__ void ext_layer(uint64_t *s)
+{
+#pragma unroll
+    for (int i = 0; i < W; i += 4) m4(s[i], s[i + 1], s[i + 2], s[i + 3]);
+    uint64_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        c[k] = s[k];
+#pragma unroll
+        for (int i = 4 + k; i < W; i += 4) c[k] = gl_add(c[k], s[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = gl_add(s[i], c[i & 3]);
+}
+
+__device__ __forceinline__ uint64_t pow7(uint64_t x)
+{
+    uint64_t x2 = gl_mul(x, x);
+    uint64_t x3 = gl_mul(x2, x);
+    uint64_t x4 = gl_mul(x2, x2);
+    return gl_mul(x3, x4);
+}
+
+template <int W>
+__device__ __forceinline__ void p2_permute(uint64_t *s)
+{
+    ext_layer<W>(s);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(r * W + i)));
+        ext_layer<W>(s);
+    }
+#pragma unroll
+    for (int r = 0; r < 22; r++) {
+        s[0] = pow7(gl_add(s[0], P2Tab<W>::rc(4 * W + r)));
+        uint64_t sum = s[0];
+#pragma unroll
+        for (int i = 1; i < W; i++) sum = gl_add(sum, s[i]);
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = gl_add(gl_mul(s[i], P2Tab<W>::d(i)), sum);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(4 * W + 22 + r * W + i)));
+        ext_layer<W>(s);
+    }
+}
+
+// ------------------------------------------------------------ leaves ----
+// Row r of the source: element c at src[r*rs + c*cs]. Digest to out[r*4 .. r*4+3].
+template <int W>
+__global__ __launch_bounds__(P2_THREADS) void k_p2_leaves(const uint64_t *__restrict__ src, uint64_t rs, uint64_t cs,
+                                                          uint64_t ncols, uint64_t nrows, uint64_t *__restrict__ out)
+{
+    constexpr int RATE = W - 4;
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    uint64_t s[W];
+    const uint64_t *row = src + r * rs;
+#pragma unroll
+    for (int i = RATE; i < W; i++) s[i] = 0;
+    if (ncols == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[r * 4 + i] = 0;
+        return;
+    }
+    for (uint64_t c0 = 0; c0 < ncols; c0 += RATE) {
+        if (c0 > 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) s[RATE + i] = s[i];
+        }
+        if (c0 + RATE <= ncols) {
+#pragma unroll
+            for (int i = 0; i < RATE; i++) s[i] = gl_canon(row[(c0 + i) * cs]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < RATE; i++) s[i] = (c0 + i < ncols) ? gl_canon(row[(c0 + i) * cs]) : 0;
+        }
+        p2_permute<W>(s);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[r * 4 + i] = s[i];
+}
+
+// ------------------------------------------------------------- nodes ----
+// One level: children = `pending` digests at in (pad digests beyond it are
+// zero), parents written to out (= in + (pending + pad)*4). The same kernel
+// also writes the zero pad digests after the children.
+template <int W>
+__global__ __launch_bounds__(P2_THREADS) void k_p2_nodes(uint64_t *__restrict__ tree, uint64_t in_off, uint64_t pending,
+                                                         uint64_t pad, uint64_t nnext)
+{
+    constexpr int ARITY = W / 4;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnext) return;
+    uint64_t s[W];
+    const uint64_t *ch = tree + in_off + i * W;
+#pragma unroll
+    for (int a = 0; a < ARITY; a++) {
+        bool real = i * ARITY + a < pending;
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[a * 4 + k] = real ? ch[a * 4 + k] : 0;
+    }
+    if (i == nnext - 1) {
+        for (uint64_t z = 0; z < pad * 4; z++) tree[in_off + pending * 4 + z] = 0;
+    }
+    p2_permute<W>(s);
+    uint64_t *o = tree + in_off + (pending + pad) * 4 + i * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = s[k];
+}
+
+// Single permutation over `count` states stored consecutively (W words each).
+template <int W>
+__global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint64_t s[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) s[k] = gl_canon(st[i * W + k]);
+    p2_permute<W>(s);
+#pragma unroll
+    for (int k = 0; k < W; k++) st[i * W + k] = s[k];
+}
+
+// ------------------------------------------------------------ launchers ----
+extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
+                                uint64_t nrows, uint64_t *out, hipStream_t st)
+{
+    if (nrows == 0) return 0;
+    dim3 g((unsigned)((nrows + P2_THREADS - 1) / P2_THREADS));
+    switch (width) {
+    case 8: hipLaunchKernelGGL(k_p2_leaves<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+    case 12: hipLaunchKernelGGL(k_p2_leaves<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+    case 16: hipLaunchKernelGGL(k_p2_leaves<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st)
+{
+    uint64_t arity = width / 4;
+    uint64_t pending = nleaves, off = 0;
+    while (pending > 1) {
+        uint64_t pad = (arity - pending % arity) % arity;
+        uint64_t next = (pending + arity - 1) / arity;
+        dim3 g((unsigned)((next + P2_THREADS - 1) / P2_THREADS));
+        switch (width) {
+        case 8: hipLaunchKernelGGL(k_p2_nodes<8>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+        case 12: hipLaunchKernelGGL(k_p2_nodes<12>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+        case 16: hipLaunchKernelGGL(k_p2_nodes<16>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+        default: return -1;
+        }
+        off += (pending + pad) * 4;
+        pending = next;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st)
+{
+    if (count == 0) return 0;
+    dim3 g((unsigned)((count + 127) / 128));
+    switch (width) {
+    case 8: hipLaunchKernelGGL(k_p2_permute_many<8>, g, dim3(128), 0, st, states, count); break;
+    case 12: hipLaunchKernelGGL(k_p2_permute_many<12>, g, dim3(128), 0, st, states, count); break;
+    case 16: hipLaunchKernelGGL(k_p2_permute_many<16>, g, dim3(128), 0, st, states, count); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,413 @@
+// Host side of libpil2_commit_hip.so: the C-ABI declared in include/pil2_commit_hip.h.
+//
+// Drop-in entry points keep the reference's names and argument meaning
+// (pil2-stark/src/api/starks_api.hpp); the device-resident pil2hip_* entry points
+// run the same operations on HBM buffers on a caller stream.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pil2_commit_hip.h"
+#include "fri_kernels.h"
+#include "gl_field.h"
+#include "ntt_kernels.h"
+#include "poseidon2_kernels.h"
+
+// ------------------------------------------------------------------ state --
+static std::mutex g_mu;
+static uint8_t g_hash_family = 0;  // unset, as in the reference (hash_family.hpp:45-66)
+static bool g_abort_on_error = true;
+static thread_local std::string t_err;
+
+static int64_t fail(int64_t code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr, what)                                                                               \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return fail(-100 - (int64_t)e_, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static const uint64_t kRoots[33] = {
+    0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
+    2198989700608ULL, 4404853092538523347ULL, 6434636298004421797ULL, 4255134452441852017ULL,
+    9113133275150391358ULL, 4355325209153869931ULL, 4308460244895131701ULL,
+    7126024226993609386ULL, 1873558160482552414ULL, 8167150655112846419ULL,
+    5718075921287398682ULL, 3411401055030829696ULL, 8982441859486529725ULL,
+    1971462654193939361ULL, 6553637399136210105ULL, 8124823329697072476ULL,
+    5936499541590631774ULL, 2709866199236980323ULL, 8877499657461974390ULL,
+    3757607247483852735ULL, 4969973714567017225ULL, 2147253751702802259ULL,
+    2530564950562219707ULL, 1905180297017055339ULL, 3524815499551269279ULL,
+    7277203076849721926ULL};  // goldilocks_base_field.cpp:11-44
+
+// Per-device twiddle tables (built once per device, kept until process exit).
+static const int kMaxDevices = 16;
+static NttTablesHost g_tables[kMaxDevices];
+static bool g_tables_ready[kMaxDevices];
+
+static int64_t ensure_tables(NttTablesHost **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_tables_ready[dev]) {
+        const size_t nr = NTT_LDS_ELEMS, nt = 4096;
+        std::vector<uint64_t> h(2 * nr + 6 * nt);
+        uint64_t *rf = h.data(), *ri = rf + nr, *t = ri + nr;
+        rf[0] = ri[0] = 0;
+        for (uint32_t lh = 0; (1u << lh) < nr; lh++) {  // h = 2^lh, w_{2h} = W[lh+1]
+            uint32_t hh = 1u << lh;
+            uint64_t w = kRoots[lh + 1], wi = gl_inv(w), a = 1, b = 1;
+            for (uint32_t j = 0; j < hh; j++) {
+                rf[hh + j] = a;
+                ri[hh + j] = b;
+                a = gl_mul(a, w);
+                b = gl_mul(b, wi);
+            }
+        }
+        uint64_t w24 = kRoots[24], w24i = gl_inv(w24);
+        uint64_t *twl_f = t, *twh_f = t + nt, *twl_i = t + 2 * nt, *twh_i = t + 3 * nt, *shl = t + 4 * nt,
+                 *shh = t + 5 * nt;
+        uint64_t w24_4096 = gl_pow(w24, 4096), w24i_4096 = gl_pow(w24i, 4096), s4096 = gl_pow(7, 4096);
+        uint64_t a = 1, b = 1, c = 1, d = 1, e = 1, f = 1;
+        for (size_t j = 0; j < nt; j++) {
+            twl_f[j] = a; twh_f[j] = b; twl_i[j] = c; twh_i[j] = d; shl[j] = e; shh[j] = f;
+            a = gl_mul(a, w24); b = gl_mul(b, w24_4096);
+            c = gl_mul(c, w24i); d = gl_mul(d, w24i_4096);
+            e = gl_mul(e, 7); f = gl_mul(f, s4096);
+        }
+        uint64_t *dptr = nullptr;
+        HIP_TRY(hipMalloc(&dptr, h.size() * 8), "hipMalloc(tables)");
+        HIP_TRY(hipMemcpy(dptr, h.data(), h.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(tables)");
+        NttTablesHost &T = g_tables[dev];
+        T.roots_f = dptr;
+        T.roots_i = dptr + nr;
+        uint64_t *dt = dptr + 2 * nr;
+        T.twl_f = dt; T.twh_f = dt + nt; T.twl_i = dt + 2 * nt; T.twh_i = dt + 3 * nt; T.shl = dt + 4 * nt;
+        T.shh = dt + 5 * nt;
+        g_tables_ready[dev] = true;
+    }
+    *out = &g_tables[dev];
+    return 0;
+}
+
+// ----------------------------------------------------------- phase timing --
+// When enabled, pil2hip_commit records HIP events on the caller's stream
+// between its kernels and accumulates the elapsed time per phase:
+// 0 transpose, 1 iNTT strided pass, 2 fused LDE block, 3 NTT strided pass,
+// 4 leaf hashing, 5 tree levels. Timed commits synchronise their stream.
+static const int kPhases = 6;
+static bool g_timing = false;
+static bool g_timing_ev_ready = false;
+static hipEvent_t g_ev[kPhases + 1];
+static double g_phase_ms[kPhases];
+static uint64_t g_timed_calls = 0;
+
+static hipEvent_t *timing_events()
+{
+    if (!g_timing) return nullptr;
+    if (!g_timing_ev_ready) {
+        for (int i = 0; i <= kPhases; i++)
+            if (hipEventCreate(&g_ev[i]) != hipSuccess) return nullptr;
+        g_timing_ev_ready = true;
+    }
+    return g_ev + 1;  // ntt_launch_lde records ev[1..4]; ev[0] is the start
+}
+
+static int64_t check_family()
+{
+    if (g_hash_family == 2) return 0;
+    if (g_hash_family == 1) return fail(-5, "hash family Poseidon1 is not implemented by this library (Poseidon2 only)");
+    return fail(-6, "hash family not set: call set_hash_family(2) first");
+}
+
+static int64_t check_arity(uint64_t arity)
+{
+    if (arity < 2 || arity > 4) return fail(-7, "unsupported arity (Poseidon2 supports 2, 3, 4)");
+    return 0;
+}
+
+// ---------------------------------------------------------- entry points --
+extern "C" {
+
+void set_hash_family(uint8_t fam)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_hash_family = fam;
+}
+
+void pil2hip_set_abort_on_error(bool v) { g_abort_on_error = v; }
+
+const char *pil2hip_last_error(void) { return t_err.c_str(); }
+
+uint64_t get_num_gpus(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return (uint64_t)n;
+}
+
+bool set_gpu_mode(bool use_gpu) { return use_gpu; }
+
+struct DeviceBuffersHip {
+    int device;
+    hipStream_t stream;
+    uint32_t arity;
+    uint32_t max_n_bits_ext;
+};
+
+void *gen_device_buffers(uint32_t node_rank, uint32_t node_size, const int32_t *numa_nodes, uint32_t arity,
+                         uint32_t max_n_bits_ext)
+{
+    (void)node_rank; (void)node_size; (void)numa_nodes;
+    DeviceBuffersHip *b = new DeviceBuffersHip();
+    if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
+        delete b;
+        fail(-8, "gen_device_buffers: no HIP device");
+        return nullptr;
+    }
+    b->arity = arity;
+    b->max_n_bits_ext = max_n_bits_ext;
+    return b;
+}
+
+void free_device_buffers(void *d_buffers)
+{
+    DeviceBuffersHip *b = (DeviceBuffersHip *)d_buffers;
+    if (!b) return;
+    hipStreamSynchronize(b->stream);
+    hipStreamDestroy(b->stream);
+    delete b;
+}
+
+uint64_t pil2hip_tree_num_elements(uint64_t rows, uint64_t arity)
+{
+    if (arity < 2) return 0;
+    uint64_t total = rows, level = rows;
+    while (level > 1) {
+        total += (arity - level % arity) % arity;
+        level = (level + arity - 1) / arity;
+        total += level;
+    }
+    return total * 4;
+}
+
+int64_t pil2hip_lde(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt,
+                    uint64_t nCols, void *stream)
+{
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nCols == 0) return 0;
+    if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
+    if (!d_lde || !d_trace || !d_scratch) return fail(-13, "null buffer");
+    NttTablesHost *tb;
+    int64_t rc = ensure_tables(&tb);
+    if (rc) return rc;
+    int e = ntt_launch_lde(d_lde, d_trace, d_scratch, (uint32_t)nBits, (uint32_t)nBitsExt, nCols, tb,
+                           (hipStream_t)stream, timing_events());
+    if (e < 0) return fail(-14, "LDE plan not supported for this shape");
+    if (e) return fail(-100 - e, std::string("LDE launch: ") + hipGetErrorString((hipError_t)e));
+    return 0;
+}
+
+int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRows, uint64_t nCols, uint64_t rowStride,
+                          uint64_t colStride, uint64_t arity, void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (nRows == 0) return 0;
+    if (!d_nodes || (!d_src && nCols)) return fail(-13, "null buffer");
+    uint32_t width = (uint32_t)(4 * arity);
+    int e = p2_launch_leaves(width, d_src, rowStride, colStride, nCols, nRows, d_nodes, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "leaf hashing launch failed");
+    e = p2_launch_tree_levels(width, d_nodes, nRows, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "tree level launch failed");
+    return 0;
+}
+
+int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
+                       uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t *ev = timing_events();
+    if (ev) hipEventRecord(g_ev[0], st);
+    if ((rc = pil2hip_lde(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, stream))) return rc;
+    uint64_t NExt = 1ull << nBitsExt;
+    uint32_t width = (uint32_t)(4 * arity);
+    int e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
+    if (e) return fail(-100 - e, "leaf hashing launch failed");
+    if (ev) hipEventRecord(g_ev[5], st);
+    e = p2_launch_tree_levels(width, d_nodes, NExt, st);
+    if (e) return fail(-100 - e, "tree level launch failed");
+    if (ev) {
+        hipEventRecord(g_ev[6], st);
+        HIP_TRY(hipEventSynchronize(g_ev[6]), "timing sync");
+        for (int i = 0; i < kPhases; i++) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, g_ev[i], g_ev[i + 1]) == hipSuccess) g_phase_ms[i] += ms;
+        }
+        g_timed_calls++;
+    }
+    return 0;
+}
+
+void pil2hip_timing_enable(bool on) { g_timing = on; }
+
+uint64_t pil2hip_timing_read(double *phase_ms, bool reset)
+{
+    uint64_t n = g_timed_calls;
+    for (int i = 0; i < kPhases; i++) {
+        if (phase_ms) phase_ms[i] = g_phase_ms[i];
+        if (reset) g_phase_ms[i] = 0;
+    }
+    if (reset) g_timed_calls = 0;
+    return n;
+}
+
+int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, void *stream)
+{
+    uint64_t n = pil2hip_tree_num_elements(nRows, arity);
+    if (n < 4) return fail(-15, "empty tree");
+    HIP_TRY(hipMemcpyAsync(root, d_nodes + n - 4, 32, hipMemcpyDeviceToHost, (hipStream_t)stream), "root copy");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "root sync");
+    return 0;
+}
+
+int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t *challenge, uint64_t step,
+                         uint64_t nBitsExt, uint64_t prevBits, uint64_t currentBits, void *stream)
+{
+    if (step == 0) return 0;
+    if (!challenge) return fail(-13, "null challenge");
+    int e = fri_launch_fold(d_out, d_pol, challenge, nBitsExt, prevBits, currentBits, (hipStream_t)stream);
+    if (e == -1) return fail(-16, "invalid fold bits");
+    if (e == -2) return fail(-17, "fold ratio must be 2^1 .. 2^5");
+    if (e) return fail(-100 - e, "fold launch failed");
+    return 0;
+}
+
+int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t degree, uint64_t tBits, void *stream)
+{
+    if (tBits > 63 || (degree >> tBits) == 0 || ((degree >> tBits) << tBits) != degree)
+        return fail(-18, "degree must be a multiple of 2^tBits");
+    int e = fri_launch_transpose(d_aux, d_pol, degree, (uint32_t)tBits, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "transpose launch failed");
+    return 0;
+}
+
+int64_t pil2hip_transpose_cm2rm(uint64_t *d_out, const uint64_t *d_in, uint64_t rows, uint64_t cols,
+                                uint64_t inColStride, uint64_t outRowStride, void *stream)
+{
+    if (rows == 0 || cols == 0) return 0;
+    if (cols > 65535u * 32u) return fail(-12, "too many columns");
+    int e = ntt_launch_cm2rm(d_out, d_in, rows, cols, inColStride, outRowStride, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "transpose launch failed");
+    return 0;
+}
+
+int64_t pil2hip_p2_permute(uint64_t *d_states, uint64_t count, uint32_t width, void *stream)
+{
+    if (width != 8 && width != 12 && width != 16) return fail(-19, "width must be 8, 12 or 16");
+    int e = p2_launch_permute(width, d_states, count, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "permute launch failed");
+    return 0;
+}
+
+int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_t start, void *stream)
+{
+    int e = ntt_launch_fill_uniform(d_out, n, seed, start, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "fill launch failed");
+    return 0;
+}
+
+// ------------------------------------------------------ write_custom_commit --
+static void legacy_error(const char *where)
+{
+    if (g_abort_on_error) {
+        fprintf(stderr, "%s: %s\n", where, t_err.c_str());
+        abort();
+    }
+}
+
+static int64_t write_file(const char *path, const void *root, const uint64_t *trace, uint64_t traceN,
+                          const uint64_t *lde_rm, uint64_t ldeN, const uint64_t *nodes, uint64_t nodesN)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(-20, std::string("cannot open ") + path);
+    bool ok = fwrite(root, 1, 32, f) == 32 && fwrite(trace, 8, traceN, f) == traceN &&
+              fwrite(lde_rm, 8, ldeN, f) == ldeN && fwrite(nodes, 8, nodesN, f) == nodesN;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : fail(-21, std::string("short write to ") + path);
+}
+
+static int64_t write_custom_commit_impl(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                                        void *d_buffers, void *buffer, const char *bufferFile)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (!root || (!buffer && nCols)) return fail(-13, "null buffer");
+    uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
+    uint64_t nTree = pil2hip_tree_num_elements(NExt, arity);
+    hipStream_t st = d_buffers ? ((DeviceBuffersHip *)d_buffers)->stream : (hipStream_t)0;
+    bool want_file = bufferFile && bufferFile[0];
+    uint64_t *d_trace = nullptr, *d_scratch = nullptr, *d_lde = nullptr, *d_nodes = nullptr, *d_rm = nullptr;
+    auto cleanup = [&]() {
+        hipFree(d_trace); hipFree(d_scratch); hipFree(d_lde); hipFree(d_nodes); hipFree(d_rm);
+    };
+    uint64_t tsz = N * nCols, lsz = NExt * nCols;
+    if (hipMalloc(&d_trace, (tsz ? tsz : 1) * 8) != hipSuccess || hipMalloc(&d_scratch, (tsz ? tsz : 1) * 8) != hipSuccess ||
+        hipMalloc(&d_lde, (lsz ? lsz : 1) * 8) != hipSuccess || hipMalloc(&d_nodes, nTree * 8) != hipSuccess) {
+        cleanup();
+        return fail(-22, "write_custom_commit: device allocation failed");
+    }
+    if (tsz && hipMemcpyAsync(d_trace, buffer, tsz * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        return fail(-23, "write_custom_commit: H2D copy failed");
+    }
+    rc = pil2hip_lde(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st);
+    if (!rc) rc = pil2hip_merkelize(d_nodes, d_lde, NExt, nCols, 1, NExt, arity, st);
+    if (!rc) rc = pil2hip_get_root((uint64_t *)root, d_nodes, NExt, arity, st);
+    if (!rc && want_file) {
+        std::vector<uint64_t> lde(lsz), nodes(nTree);
+        if (lsz) {
+            if (hipMalloc(&d_rm, lsz * 8) != hipSuccess) {
+                cleanup();
+                return fail(-22, "write_custom_commit: device allocation failed");
+            }
+            rc = pil2hip_transpose_cm2rm(d_rm, d_lde, NExt, nCols, NExt, nCols, st);
+            if (!rc && hipMemcpyAsync(lde.data(), d_rm, lsz * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+                rc = fail(-24, "D2H copy failed");
+        }
+        if (!rc && hipMemcpyAsync(nodes.data(), d_nodes, nTree * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = fail(-24, "D2H copy failed");
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(-25, "stream sync failed");
+        if (!rc) rc = write_file(bufferFile, root, (const uint64_t *)buffer, tsz, lde.data(), lsz, nodes.data(), nTree);
+    }
+    hipStreamSynchronize(st);
+    cleanup();
+    return rc;
+}
+
+void write_custom_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *d_buffers,
+                         void *buffer, char *bufferFile)
+{
+    t_err.clear();
+    if (write_custom_commit_impl(root, arity, nBits, nBitsExt, nCols, d_buffers, buffer, bufferFile))
+        legacy_error("write_custom_commit");
+}
+
+}  // extern "C"

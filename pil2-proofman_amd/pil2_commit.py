@@ -1,0 +1,309 @@
+"""Host-side mirror of pil2-stark's commitment interface over libpil2_commit_hip.so.
+
+The classes keep the reference's names and argument meaning so code written
+against the C++ API reads the same:
+
+  NTT_Goldilocks.LDE         pil2-stark/src/goldilocks/src/ntt_goldilocks.cpp:369-404
+  MerkleTreeGL               pil2-stark/src/starkpil/merkleTree/merkleTreeGL.{hpp,cpp}
+                             (merkelize :316-336, getRoot :103-107, getGroupProof
+                             :145-175, getLevel :84-101, getTreeNumElements
+                             goldilocks_tooling.hpp:9-21)
+  FRI.fold / FRI.merkelize   pil2-stark/src/starkpil/fri/fri.hpp:32-109
+  write_custom_commit        pil2-stark/src/api/starks_api.hpp:96
+  set_hash_family            pil2-stark/src/api/starks_api.hpp:28
+
+Device buffers are torch tensors (dtype int64 holding the u64 bit pattern) on a
+HIP device; torch is only the allocator and stream provider. Every operation
+runs in the HIP library; if the library is missing this module raises at
+import time — there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpil2_commit_hip.so")
+P = 0xFFFFFFFF00000001
+HASH_SIZE = 4
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libpil2_commit_hip.so is not built (run `python pil2-proofman_amd/build.py` "
+        "or __graft_entry__.build()); no CPU fallback exists")
+
+_lib = ctypes.CDLL(LIB_PATH)
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_i64 = ctypes.c_int64
+
+_SIGS = {
+    "set_hash_family": (None, [ctypes.c_uint8]),
+    "write_custom_commit": (None, [_vp, _u64, _u64, _u64, _u64, _vp, _vp, ctypes.c_char_p]),
+    "gen_device_buffers": (_vp, [ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "free_device_buffers": (None, [_vp]),
+    "get_num_gpus": (_u64, []),
+    "set_gpu_mode": (ctypes.c_bool, [ctypes.c_bool]),
+    "pil2hip_tree_num_elements": (_u64, [_u64, _u64]),
+    "pil2hip_lde": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _vp]),
+    "pil2hip_merkelize": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_get_root": (_i64, [_vp, _vp, _u64, _u64, _vp]),
+    "pil2hip_fri_fold": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_fri_transpose": (_i64, [_vp, _vp, _u64, _u64, _vp]),
+    "pil2hip_transpose_cm2rm": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_p2_permute": (_i64, [_vp, _u64, ctypes.c_uint32, _vp]),
+    "pil2hip_fill_uniform": (_i64, [_vp, _u64, _u64, _u64, _vp]),
+    "pil2hip_timing_enable": (None, [ctypes.c_bool]),
+    "pil2hip_timing_read": (_u64, [_vp, ctypes.c_bool]),
+    "pil2hip_last_error": (ctypes.c_char_p, []),
+    "pil2hip_set_abort_on_error": (None, [ctypes.c_bool]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+lib = _lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise HipError("libpil2_commit_hip error %d: %s" % (rc, _lib.pil2hip_last_error().decode()))
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def to_device(arr, device="cuda"):
+    """numpy uint64 array -> int64 device tensor with the same bits."""
+    torch = _torch()
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    return torch.from_numpy(a.view(np.int64)).to(device)
+
+
+def to_host(t):
+    """device tensor (int64 bits) -> numpy uint64 array."""
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def empty(n, device="cuda"):
+    return _torch().empty(int(n), dtype=_torch().int64, device=device)
+
+
+# ------------------------------------------------------------------ ABI --
+def set_hash_family(fam):
+    """starks_api.hpp:28 — 1 = Poseidon1 (rejected by this library), 2 = Poseidon2."""
+    _lib.set_hash_family(int(fam))
+
+
+def set_abort_on_error(flag):
+    _lib.pil2hip_set_abort_on_error(bool(flag))
+
+
+def get_tree_num_elements(degree, arity=2):
+    """goldilocks_tooling.hpp:9-21"""
+    return int(_lib.pil2hip_tree_num_elements(int(degree), int(arity)))
+
+
+def write_custom_commit(root, arity, n_bits, n_bits_ext, n_cols, d_buffers, buffer, buffer_file=""):
+    """starks_api.hpp:96. root: numpy uint64[4] (out); buffer: numpy row-major trace."""
+    assert root.dtype == np.uint64 and root.size >= 4
+    buf = np.ascontiguousarray(buffer, dtype=np.uint64)
+    _lib.write_custom_commit(root.ctypes.data_as(_vp), arity, n_bits, n_bits_ext, n_cols,
+                             d_buffers, buf.ctypes.data_as(_vp), buffer_file.encode())
+    err = _lib.pil2hip_last_error().decode()
+    if err:
+        raise HipError("write_custom_commit: " + err)
+
+
+def gen_device_buffers(arity=3, max_n_bits_ext=24):
+    return _lib.gen_device_buffers(0, 1, None, arity, max_n_bits_ext)
+
+
+def free_device_buffers(handle):
+    _lib.free_device_buffers(handle)
+
+
+# ----------------------------------------------------------------- NTT --
+class NTT_Goldilocks:
+    """Mirror of NTT_Goldilocks (ntt_goldilocks.hpp:66-193) for the LDE entry point."""
+
+    def __init__(self, max_domain_size):
+        self.n = int(max_domain_size)
+
+    @staticmethod
+    def _log2(n):
+        lg = int(n).bit_length() - 1
+        if (1 << lg) != n:
+            raise ValueError("size must be a power of two")
+        return lg
+
+    def LDE(self, output, input, n_extended, n, ncols, buffer=None, stream=None):
+        """ntt_goldilocks.cpp:369-404. input: device row-major n x ncols; output:
+        device column-major ncols x n_extended; buffer: device scratch >= n*ncols."""
+        if n == 0 or ncols == 0:
+            return
+        if buffer is None:
+            buffer = empty(n * ncols, device=input.device)
+        _check(_lib.pil2hip_lde(_ptr(output), _ptr(input), _ptr(buffer), self._log2(n),
+                                self._log2(n_extended), ncols, _stream(stream)))
+
+
+# ---------------------------------------------------------------- tree --
+class MerkleTreeGL:
+    """Mirror of MerkleTreeGL over device buffers (merkleTreeGL.hpp:12-115).
+
+    source: device tensor; element (row, col) at source[row*row_stride + col*col_stride]
+    (column-major LDE: row_stride=1, col_stride=height; row-major: width, 1).
+    """
+
+    def __init__(self, arity, last_level_verification, custom, height, width,
+                 source=None, row_stride=None, col_stride=None, nodes=None):
+        self.arity = int(arity)
+        self.last_level_verification = int(last_level_verification)
+        self.custom = custom
+        self.height = int(height)
+        self.width = int(width)
+        self.num_nodes = get_tree_num_elements(self.height, self.arity)
+        self.source = source
+        self.row_stride = self.width if row_stride is None else int(row_stride)
+        self.col_stride = 1 if col_stride is None else int(col_stride)
+        self.nodes = nodes
+
+    def set_source(self, source, row_stride=None, col_stride=None):
+        self.source = source
+        if row_stride is not None:
+            self.row_stride, self.col_stride = int(row_stride), int(col_stride)
+
+    def set_nodes(self, nodes):
+        self.nodes = nodes
+
+    def merkelize(self, stream=None):
+        """merkleTreeGL.cpp:316-336"""
+        if self.nodes is None:
+            self.nodes = empty(self.num_nodes, device=self.source.device)
+        _check(_lib.pil2hip_merkelize(_ptr(self.nodes), _ptr(self.source), self.height, self.width,
+                                      self.row_stride, self.col_stride, self.arity, _stream(stream)))
+
+    def get_root(self, stream=None):
+        """merkleTreeGL.cpp:103-107"""
+        root = np.zeros(4, dtype=np.uint64)
+        _check(_lib.pil2hip_get_root(root.ctypes.data_as(_vp), _ptr(self.nodes), self.height,
+                                     self.arity, _stream(stream)))
+        return root
+
+    def get_merkle_proof_length(self):
+        """merkleTreeGL.cpp:50-55"""
+        if self.height > 1:
+            import math
+            return int(math.ceil(math.log2(self.height) / math.log2(self.arity))) - self.last_level_verification
+        return 0
+
+    def get_level(self):
+        """merkleTreeGL.cpp:84-101: the nodes of the level with <= arity^llv entries,
+        zero-padded to arity^llv digests."""
+        llv = self.last_level_verification
+        if llv == 0:
+            return np.zeros(0, dtype=np.uint64)
+        nodes = to_host(self.nodes)
+        n, off, lim = self.height, 0, self.arity ** llv
+        while n > lim:
+            n = (n + self.arity - 1) // self.arity
+            off += n * self.arity * HASH_SIZE
+        out = np.zeros(lim * HASH_SIZE, dtype=np.uint64)
+        out[:n * HASH_SIZE] = nodes[off:off + n * HASH_SIZE]
+        return out
+
+    def get_group_proof(self, idx):
+        """merkleTreeGL.cpp:145-175: row values then sibling digests per level."""
+        torch = _torch()
+        row_idx = torch.arange(self.width, device=self.source.device, dtype=torch.int64) * self.col_stride
+        vals = to_host(self.source.reshape(-1)[idx * self.row_stride + row_idx])
+        nodes = to_host(self.nodes)
+        out = [vals]
+        n, offset, llv, a = self.height, 0, self.last_level_verification, self.arity
+        lim = a ** llv
+        while (n > 1) if llv == 0 else (n > lim):
+            cur = idx % a
+            si = idx - cur
+            for i in range(a):
+                if i != cur:
+                    out.append(nodes[(offset + si + i) * 4:(offset + si + i) * 4 + 4])
+            nxt = (n + a - 1) // a
+            offset += nxt * a
+            idx //= a
+            n = nxt
+        return np.concatenate(out)
+
+
+# ----------------------------------------------------------------- FRI --
+class FRI:
+    """Mirror of FRI<Goldilocks::Element> (fri.hpp) fold and per-layer commit."""
+
+    @staticmethod
+    def fold(step, pol, challenge, n_bits_ext, prev_bits, current_bits, out=None, stream=None):
+        """fri.hpp:32-99. pol: device tensor of 2^prev_bits cubic values (3 words each).
+        Folds in place unless `out` is given; returns the tensor holding the result."""
+        ch = np.ascontiguousarray(challenge, dtype=np.uint64)
+        dst = pol if out is None else out
+        _check(_lib.pil2hip_fri_fold(_ptr(dst), _ptr(pol), ch.ctypes.data_as(_vp), step, n_bits_ext,
+                                     prev_bits, current_bits, _stream(stream)))
+        return dst
+
+    @staticmethod
+    def merkelize(tree, pol, current_bits, next_bits, stream=None):
+        """fri.hpp:102-109: regroup (getTransposed) into tree.source, merkelize, root."""
+        degree = 1 << current_bits
+        if tree.source is None:
+            tree.source = empty(degree * 3, device=pol.device)
+        _check(_lib.pil2hip_fri_transpose(_ptr(tree.source), _ptr(pol), degree, next_bits, _stream(stream)))
+        tree.row_stride, tree.col_stride = tree.width, 1
+        tree.merkelize(stream)
+        return tree.get_root(stream)
+
+
+def p2_permute(states, width, stream=None):
+    """Poseidon2Goldilocks<width>::permute over consecutive states (device tensor)."""
+    count = states.numel() // width
+    _check(_lib.pil2hip_p2_permute(_ptr(states), count, width, _stream(stream)))
+
+
+def fill_uniform(out, seed, start=0, stream=None):
+    """Fill a device tensor with the seeded uniform stream (datagen.uniform on device)."""
+    _check(_lib.pil2hip_fill_uniform(_ptr(out), out.numel(), seed, start, _stream(stream)))
+    return out
+
+
+def transpose_cm2rm(out, inp, rows, cols, in_col_stride, out_row_stride, stream=None):
+    _check(_lib.pil2hip_transpose_cm2rm(_ptr(out), _ptr(inp), rows, cols, in_col_stride,
+                                        out_row_stride, _stream(stream)))
+
+
+def commit(trace_dev, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, scratch=None, stream=None):
+    """LDE + merkelize on device (commit_witness_cpu core, starks_api.cpp:683-723).
+    Returns (lde column-major, nodes)."""
+    n, ne = 1 << n_bits, 1 << n_bits_ext
+    dev = trace_dev.device
+    lde = empty(ne * n_cols, dev) if lde is None else lde
+    nodes = empty(get_tree_num_elements(ne, arity), dev) if nodes is None else nodes
+    scratch = empty(max(n * n_cols, 1), dev) if scratch is None else scratch
+    _check(_lib.pil2hip_commit(_ptr(lde), _ptr(nodes), _ptr(trace_dev), _ptr(scratch), arity, n_bits,
+                               n_bits_ext, n_cols, _stream(stream)))
+    return lde, nodes

@@ -1,0 +1,281 @@
+"""GPU parity tests: libpil2_commit_hip.so (through its C-ABI) against the oracle
+and the fixtures produced by the compiled reference. Integer work, so every
+comparison is exact (canonical values)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import hexes
+from golden_helpers import colmajor_to_rowmajor, lde_digest
+import datagen
+
+pytestmark = pytest.mark.gpu
+P = 0xFFFFFFFF00000001
+
+
+@pytest.fixture(scope="module")
+def pc():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a HIP device")
+    import pil2_commit
+    pil2_commit.set_hash_family(2)
+    pil2_commit.set_abort_on_error(False)
+    return pil2_commit
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def gpu_lde(pc, trace, nb, nbe):
+    n, c = trace.shape
+    out = pc.empty((1 << nbe) * c)
+    pc.NTT_Goldilocks(n).LDE(out, pc.to_device(trace), 1 << nbe, n, c)
+    _sync()
+    return colmajor_to_rowmajor(pc.to_host(out), 1 << nbe, c)
+
+
+# ------------------------------------------------------------- Poseidon2 --
+@pytest.mark.parametrize("w", [8, 12, 16])
+def test_permute_reference_goldens(pc, refvec, w):
+    st = pc.to_device(np.arange(w, dtype=np.uint64))
+    pc.p2_permute(st, w)
+    _sync()
+    assert [int(v) for v in pc.to_host(st)] == hexes(refvec["permute_w%d" % w])
+
+
+def test_permute_random_batch(pc, oracle):
+    for w in (8, 12, 16):
+        states = datagen.uniform(w * 1000, w)
+        d = pc.to_device(states)
+        pc.p2_permute(d, w)
+        _sync()
+        got = pc.to_host(d).reshape(1000, w)
+        for i in (0, 1, 500, 999):
+            assert (got[i] == oracle.permute(states[i * w:(i + 1) * w])).all()
+
+
+# ------------------------------------------------------------------- LDE --
+LDE_SHAPES = [(0, 1, 4), (1, 1, 3), (1, 3, 2), (3, 5, 5), (4, 5, 8), (6, 8, 12), (10, 12, 1), (12, 13, 8),
+              (12, 12, 7), (13, 14, 32), (11, 13, 3), (14, 16, 5), (15, 16, 4), (16, 17, 32), (17, 19, 3),
+              (18, 19, 2), (19, 20, 1), (20, 22, 1)]
+
+
+@pytest.mark.parametrize("nb,nbe,c", LDE_SHAPES)
+def test_lde_matches_oracle(pc, oracle, nb, nbe, c):
+    print("lde case", nb, nbe, c, flush=True)
+    trace = datagen.uniform((1 << nb) * c, 0xC0FFEE + nb * 100 + nbe * 10 + c).reshape(1 << nb, c)
+    got = gpu_lde(pc, trace, nb, nbe)
+    assert (got == oracle.lde(trace, 1 << nbe)).all()
+
+
+def test_lde_noncanonical_input(pc, oracle):
+    nb, nbe, c = 8, 9, 4
+    trace = datagen.uniform((1 << nb) * c, 5).reshape(1 << nb, c)
+    small = trace < np.uint64(2**32 - 1)
+    raw = trace.copy()
+    raw[small] += np.uint64(P)  # same residue, non-canonical word
+    assert (gpu_lde(pc, raw, nb, nbe) == oracle.lde(trace, 1 << nbe)).all()
+
+
+# ---------------------------------------------------------------- commit --
+def test_commit_roots_match_compiled_reference(pc, compiled_ref):
+    for case in compiled_ref["cases"]:
+        nb, nbe, c = case["n_bits"], case["n_bits_ext"], case["n_cols"]
+        print("fixture case", nb, nbe, c, flush=True)
+        trace = datagen.uniform((1 << nb) * c, case["seed"]).reshape(1 << nb, c)
+        d_trace = pc.to_device(trace)
+        for a, root in case["roots"].items():
+            lde, nodes = pc.commit(d_trace, int(a), nb, nbe, c)
+            t = pc.MerkleTreeGL(int(a), 0, True, 1 << nbe, c, source=lde, row_stride=1, col_stride=1 << nbe,
+                                nodes=nodes)
+            assert [int(v) for v in t.get_root()] == hexes(root), (nb, nbe, c, a)
+        got = colmajor_to_rowmajor(pc.to_host(lde), 1 << nbe, c)
+        assert lde_digest(got) == case["lde_sha256"]
+        assert [int(v) for v in got[0]] == hexes(case["lde_first_row"])
+
+
+@pytest.mark.parametrize("arity", [2, 3, 4])
+@pytest.mark.parametrize("rows,cols", [(1, 5), (2, 3), (3, 8), (5, 9), (7, 0), (100, 17), (1000, 12), (4096, 1),
+                                       (2187, 24)])
+def test_merkelize_rowmajor_full_tree(pc, oracle, arity, rows, cols):
+    src = datagen.uniform(max(rows * cols, 1), rows * 31 + cols)[:rows * cols].reshape(rows, cols)
+    d = pc.to_device(src if src.size else np.zeros(1, np.uint64))
+    t = pc.MerkleTreeGL(arity, 0, True, rows, cols, source=d)
+    t.merkelize()
+    _sync()
+    assert (pc.to_host(t.nodes) == oracle.merkletree(src, arity)).all()
+
+
+def test_commit_is_deterministic(pc):
+    nb, nbe, c = 14, 16, 24
+    d = pc.empty((1 << nb) * c)
+    pc.fill_uniform(d, 0x5EED0001)
+    roots = []
+    for _ in range(3):
+        lde, nodes = pc.commit(d, 3, nb, nbe, c)
+        roots.append(pc.MerkleTreeGL(3, 0, True, 1 << nbe, c, nodes=nodes).get_root())
+    assert all((r == roots[0]).all() for r in roots)
+
+
+def test_fill_uniform_matches_datagen(pc):
+    d = pc.empty(100003)
+    pc.fill_uniform(d, 0x5EED0002, start=11)
+    _sync()
+    assert (pc.to_host(d) == datagen.uniform(100003, 0x5EED0002, start=11)).all()
+
+
+# ---------------------------------------------------- write_custom_commit --
+def test_write_custom_commit_plain_small(pc, compiled_ref, oracle, tmp_path):
+    case = compiled_ref["cases"][0]
+    nb, nbe, c = case["n_bits"], case["n_bits_ext"], case["n_cols"]
+    assert (nb, nbe, c) == (16, 17, 32)
+    trace = datagen.uniform((1 << nb) * c, case["seed"]).reshape(1 << nb, c)
+    for a in ("3", "4"):
+        root = np.zeros(4, dtype=np.uint64)
+        f = str(tmp_path / ("cc_%s.bin" % a))
+        pc.write_custom_commit(root, int(a), nb, nbe, c, None, trace, f)
+        assert [int(v) for v in root] == hexes(case["roots"][a])
+        raw = np.fromfile(f, dtype="<u8")
+        n, ne = 1 << nb, 1 << nbe
+        ntree = pc.get_tree_num_elements(ne, int(a))
+        assert raw.size == 4 + n * c + ne * c + ntree
+        assert (raw[:4] == root).all()
+        assert (raw[4:4 + n * c] == trace.ravel()).all()
+        lde = raw[4 + n * c:4 + n * c + ne * c].reshape(ne, c)
+        assert lde_digest(lde) == case["lde_sha256"]
+        assert (raw[4 + n * c + ne * c:][-4:] == root).all()
+
+
+def test_write_custom_commit_with_device_buffers_handle(pc, compiled_ref):
+    case = compiled_ref["cases"][1]
+    nb, nbe, c = case["n_bits"], case["n_bits_ext"], case["n_cols"]
+    trace = datagen.uniform((1 << nb) * c, case["seed"]).reshape(1 << nb, c)
+    h = pc.gen_device_buffers(3, nbe)
+    try:
+        root = np.zeros(4, dtype=np.uint64)
+        pc.write_custom_commit(root, 3, nb, nbe, c, h, trace, "")
+        assert [int(v) for v in root] == hexes(case["roots"]["3"])
+    finally:
+        pc.free_device_buffers(h)
+
+
+def test_error_paths_report_instead_of_abort(pc):
+    trace = np.zeros((16, 2), dtype=np.uint64)
+    root = np.zeros(4, dtype=np.uint64)
+    try:
+        pc.set_hash_family(1)
+        with pytest.raises(pc.HipError, match="Poseidon1"):
+            pc.write_custom_commit(root, 3, 4, 5, 2, None, trace, "")
+        pc.set_hash_family(2)
+        with pytest.raises(pc.HipError, match="arity"):
+            pc.write_custom_commit(root, 5, 4, 5, 2, None, trace, "")
+        with pytest.raises(pc.HipError, match="nBitsExt"):
+            pc.write_custom_commit(root, 3, 5, 4, 2, None, trace, "")
+    finally:
+        pc.set_hash_family(2)
+
+
+# ------------------------------------------------------------------- FRI --
+def fri_steps(nbe, folding=3, final_degree=5):
+    """FRI step schedule (setup/pil2-stark/src/types/stark_struct.rs:162-166)."""
+    steps = [nbe]
+    b = nbe
+    while b > final_degree + 1:
+        b = b - folding if b > folding + final_degree else final_degree
+        steps.append(b)
+    return steps
+
+
+@pytest.mark.parametrize("nbe", [12, 16, 22])
+def test_fri_fold_all_layers(pc, oracle, nbe):
+    """FRI config (seed 0x5EED0004): fold + per-layer Merkle commit, all layers and
+    all roots exact vs the oracle. nbe=22 is the full FRI fold configuration."""
+    steps = fri_steps(nbe)
+    if nbe == 22:
+        assert steps == [22, 19, 16, 13, 10, 7, 5]
+    pol = datagen.uniform(3 << nbe, 0x5EED0004)
+    chal = datagen.challenges(len(steps), 0x5EED0004 + 1)
+    d = pc.to_device(pol)
+    host = pol.copy()
+    for s in range(1, len(steps)):
+        prev, cur = steps[s - 1], steps[s]
+        print("fri step", s, prev, "->", cur, flush=True)
+        pc.FRI.fold(s, d, chal[s], nbe, prev, cur)
+        host = oracle.fri_fold(host, chal[s], s, nbe, prev, cur)
+        _sync()
+        got = pc.to_host(d)[:3 << cur]
+        assert (got == host).all(), "layer %d" % s
+        if s < len(steps) - 1:
+            nxt = steps[s + 1]
+            w = 3 << (cur - nxt)
+            tree = pc.MerkleTreeGL(3, 2, False, 1 << nxt, w)
+            root = pc.FRI.merkelize(tree, d, cur, nxt)
+            aux = oracle.fri_transpose(host, 1 << cur, nxt)
+            expect = oracle.merkletree(aux.reshape(1 << nxt, w), 3)
+            assert (root == expect[-4:]).all(), "root %d" % s
+            assert (pc.to_host(tree.source) == aux).all()
+
+
+# ------------------------------------------------------------- openings --
+@pytest.mark.parametrize("arity,llv", [(3, 2), (4, 2), (2, 0)])
+def test_group_proofs_match_oracle(pc, oracle, arity, llv):
+    rows, cols = 1 << 10, 7
+    src = datagen.uniform(rows * cols, 99).reshape(rows, cols)
+    t = pc.MerkleTreeGL(arity, llv, True, rows, cols, source=pc.to_device(src))
+    t.merkelize()
+    tree = oracle.merkletree(src, arity)
+    for idx in (0, 5, 511, rows - 1):
+        assert (t.get_group_proof(idx) == oracle.merkle_open(tree, src, arity, llv, idx)).all()
+
+
+# ------------------------------------------------------- large (sampled) --
+def test_plain_large_commit_sampled(pc, oracle):
+    """Plain-large configuration 2^22 x 128, blowup 4, arity 3 (seed 0x5EED0002).
+    Size-independent checks: two full columns of the LDE equal the oracle's LDE of
+    those columns; sampled leaf digests equal the oracle's sponge over the GPU's
+    LDE rows; sampled nodes on every level equal the permutation of their
+    children; the root is reproducible."""
+    import torch
+    nb, nbe, c, arity = 22, 24, 128, 3
+    n, ne = 1 << nb, 1 << nbe
+    d_trace = pc.empty(n * c)
+    pc.fill_uniform(d_trace, 0x5EED0002)
+    print("large: commit", flush=True)
+    lde, nodes = pc.commit(d_trace, arity, nb, nbe, c)
+    root1 = pc.MerkleTreeGL(arity, 0, True, ne, c, nodes=nodes).get_root()
+    tr = d_trace.view(n, c)
+    for col in (0, 77):
+        print("large: column", col, flush=True)
+        colv = pc.to_host(tr[:, col].contiguous())
+        expect = oracle.lde(colv.reshape(n, 1), ne)[:, 0]
+        assert (pc.to_host(lde[col * ne:(col + 1) * ne]) == expect).all(), col
+    rng = np.random.default_rng(7)
+    rows = np.concatenate([[0, 1, ne - 1], rng.integers(0, ne, 40)])
+    L = lde.view(c, ne)
+    idx = torch.from_numpy(rows.astype(np.int64)).to(lde.device)
+    sampled = pc.to_host(L[:, idx].t().contiguous())
+    nd = nodes
+    for k, r in enumerate(rows):
+        dig = pc.to_host(nd[int(r) * 4:int(r) * 4 + 4])
+        assert (dig == oracle.linear_hash(sampled[k], 12)).all(), r
+    # every level: sampled parents = permute(children)
+    pending, off = ne, 0
+    while pending > 1:
+        pad = (arity - pending % arity) % arity
+        nxt = (pending + arity - 1) // arity
+        for i in sorted(set([0, nxt - 1, int(rng.integers(0, nxt))])):
+            ch = np.zeros(12, dtype=np.uint64)
+            lo = (off + i * arity) * 4
+            avail = min(arity, pending - i * arity) * 4
+            ch[:avail] = pc.to_host(nd[lo:lo + avail])
+            par = pc.to_host(nd[(off + pending + pad + i) * 4:(off + pending + pad + i) * 4 + 4])
+            assert (par == oracle.permute(ch)[:4]).all()
+        off += pending + pad
+        pending = nxt
+    lde2, nodes2 = pc.commit(d_trace, arity, nb, nbe, c, lde=lde, nodes=nodes)
+    root2 = pc.MerkleTreeGL(arity, 0, True, ne, c, nodes=nodes2).get_root()
+    assert (root1 == root2).all()

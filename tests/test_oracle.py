@@ -1,0 +1,265 @@
+"""CPU tests that pin the oracle (oracle/gl_oracle.c) before it is trusted.
+
+Each oracle function is checked against something it does not itself restate:
+the reference's own golden vectors, the compiled reference (oracle/_ref, where
+built), or what the mathematics fixes (128-bit modular reduction, root-of-unity
+chain, brute-force DFT, closed-form polynomial evaluation, per-coset
+interpolation for the FRI fold, Merkle path recomputation).
+"""
+import random
+
+import numpy as np
+import pytest
+
+from conftest import hexes
+import datagen
+
+P = 0xFFFFFFFF00000001
+
+
+# ------------------------------------------------------------------ field --
+def test_mul_matches_128bit_modulo(oracle):
+    rng = random.Random(1)
+    edge = [0, 1, 2, P - 1, P - 2, 2**32, 2**32 - 1, 2**63, 2**64 - 1, P, P + 1]
+    vals = edge + [rng.getrandbits(64) for _ in range(2000)]
+    for i, a in enumerate(vals):
+        b = vals[(i * 7 + 3) % len(vals)]
+        assert oracle.mul(a, b) == (a * b) % P
+        assert oracle.add(a, b) == (a + b) % P
+        assert oracle.sub(a, b) == (a - b) % P
+
+
+def test_inverse_and_roots(oracle):
+    for a in [1, 2, 7, P - 1, 123456789]:
+        assert oracle.mul(a, oracle.inv(a)) == 1
+    # W[k]^2 == W[k-1], W[1] == -1, W[32] has order exactly 2^32
+    for k in range(1, 33):
+        assert oracle.mul(oracle.root(k), oracle.root(k)) == oracle.root(k - 1)
+    assert oracle.root(1) == P - 1
+    assert pow(oracle.root(32), 2**31, P) == P - 1
+
+
+def test_cubic_mul_identities(oracle):
+    rng = random.Random(2)
+    x = [0, 1, 0]
+    x2 = oracle.c3_mul(x, x)
+    assert list(x2) == [0, 0, 1]
+    assert list(oracle.c3_mul(x2, x)) == [1, 1, 0]  # x^3 = x + 1
+    for _ in range(50):
+        a, b, c = ([rng.randrange(P) for _ in range(3)] for _ in range(3))
+        ab_c = oracle.c3_mul(oracle.c3_mul(a, b), c)
+        a_bc = oracle.c3_mul(a, oracle.c3_mul(b, c))
+        assert (ab_c == a_bc).all()
+        assert (oracle.c3_mul(a, b) == oracle.c3_mul(b, a)).all()
+
+
+# -------------------------------------------------------------------- NTT --
+@pytest.mark.parametrize("logn", [0, 1, 2, 3, 5, 8])
+def test_ntt_matches_bruteforce_dft(oracle, logn):
+    a = datagen.uniform(1 << logn, 10 + logn)
+    assert (oracle.ntt(a) == oracle.dft_naive(a)).all()
+    assert (oracle.intt(a) == oracle.dft_naive(a, inverse=True)).all()
+    assert (oracle.intt(oracle.ntt(a)) == a).all()
+
+
+def test_lde_reference_golden_extendePol(oracle, refvec):
+    """TEST(GOLDILOCKS_TEST, extendePol), test_ntt_cpu.cpp:351-412."""
+    a = np.zeros((16, 8), dtype=object)
+    for j in range(8):
+        a[0, j] = a[1, j] = 1 + j
+    for i in range(2, 16):
+        a[i] = (a[i - 1] + a[i - 2]) % P
+    out = oracle.lde(a.astype(np.uint64), 32)
+    assert [int(v) for v in out[:, 0]] == hexes(refvec["extendePol_16x8_to_32_col0"])
+
+
+@pytest.mark.parametrize("n,next_,ncols", [(16, 64, 1), (16, 32, 4), (64, 256, 8)])
+def test_lde_closed_form(oracle, n, next_, ncols):
+    """TEST(GOLDILOCKS_TEST, LDE_correctness), test_ntt_cpu.cpp:415-471:
+    out[j][c] = p_c(7 * w_NExt^j) for p_c given by its coefficients."""
+    coeff = np.array([[((k * ncols + c + 1) * 1000003) % P for c in range(ncols)] for k in range(n)],
+                     dtype=np.uint64)
+    inp = np.stack([oracle.ntt(coeff[:, c]) for c in range(ncols)], axis=1)
+    out = oracle.lde(inp, next_)
+    w = oracle.root(next_.bit_length() - 1)
+    for j in range(0, next_, max(1, next_ // 16)):
+        x = (7 * pow(w, j, P)) % P
+        for c in range(ncols):
+            assert int(out[j, c]) == oracle.poly_eval(coeff[:, c], x)
+
+
+def test_lde_matches_compiled_reference_fixtures(oracle, compiled_ref):
+    from golden_helpers import lde_digest
+    for case in compiled_ref["cases"]:
+        n, ne, c = 1 << case["n_bits"], 1 << case["n_bits_ext"], case["n_cols"]
+        if n * c > (1 << 19):
+            continue
+        trace = datagen.uniform(n * c, case["seed"]).reshape(n, c)
+        out = oracle.lde(trace, ne)
+        assert lde_digest(out) == case["lde_sha256"], case
+
+
+# -------------------------------------------------------------- Poseidon2 --
+@pytest.mark.parametrize("w", [8, 12, 16])
+def test_permute_reference_goldens(oracle, refvec, w):
+    st = oracle.permute(np.arange(w, dtype=np.uint64))
+    assert [int(v) for v in st] == hexes(refvec["permute_w%d" % w])
+
+
+@pytest.mark.parametrize("w", [8, 12, 16])
+def test_linear_hash_reference_goldens(oracle, refvec, w):
+    out = oracle.linear_hash(np.arange(1, 21, dtype=np.uint64), w)
+    assert [int(v) for v in out] == hexes(refvec["linear_hash_w%d_1to20" % w])
+
+
+@pytest.mark.parametrize("w", [8, 12, 16])
+def test_linear_hash_edges(oracle, w):
+    assert list(oracle.linear_hash(np.zeros(0, dtype=np.uint64), w)) == [0, 0, 0, 0]
+    # one block: equals permute of the zero-padded rate with zero capacity
+    v = np.array([42], dtype=np.uint64)
+    st = np.zeros(w, dtype=np.uint64)
+    st[0] = 42
+    assert (oracle.linear_hash(v, w) == oracle.permute(st)[:4]).all()
+
+
+def test_merkletree_reduce_golden(oracle, refvec):
+    """merkletreeReduce golden (test_helpers.hpp:62-63): 3 digests -> one node."""
+    src = np.arange(1, 13, dtype=np.uint64)
+    root = oracle.permute(src)[:4]
+    assert [int(v) for v in root] == hexes(refvec["merkletree_reduce_w12_ar3_1to12"])
+
+
+@pytest.mark.parametrize("arity", [2, 3, 4])
+def test_merkletree_single_row_is_linear_hash(oracle, arity):
+    row = np.array([i * 7 + 3 for i in range(4 * arity)], dtype=np.uint64)
+    tree = oracle.merkletree(row.reshape(1, -1), arity)
+    assert (tree[-4:] == oracle.linear_hash(row, 4 * arity)).all()
+
+
+def test_merkletree_roots_match_compiled_reference(oracle, compiled_ref):
+    for case in compiled_ref["cases"]:
+        n, ne, c = 1 << case["n_bits"], 1 << case["n_bits_ext"], case["n_cols"]
+        if ne * c > (1 << 20):
+            continue
+        trace = datagen.uniform(n * c, case["seed"]).reshape(n, c)
+        lde = oracle.lde(trace, ne)
+        for a, root in case["roots"].items():
+            tree = oracle.merkletree(lde, int(a))
+            assert [int(v) for v in tree[-4:]] == hexes(root), (case["n_bits"], case["n_cols"], a)
+
+
+@pytest.mark.parametrize("arity,llv", [(2, 0), (3, 2), (4, 2), (3, 0)])
+def test_merkle_openings_recompute_root(oracle, arity, llv):
+    rows, cols = 300, 5
+    src = datagen.uniform(rows * cols, 77).reshape(rows, cols)
+    tree = oracle.merkletree(src, arity)
+    # level offsets
+    for idx in [0, 1, 17, rows - 1]:
+        proof = oracle.merkle_open(tree, src, arity, llv, idx)
+        assert (proof[:cols] == src[idx]).all()
+        h = oracle.linear_hash(src[idx], 4 * arity)
+        pos, k, i = idx, cols, 0
+        n = rows
+        lim = arity ** llv
+        while (n > 1) if llv == 0 else (n > lim):
+            cur = pos % arity
+            st = np.zeros(4 * arity, dtype=np.uint64)
+            sib = 0
+            for j in range(arity):
+                if j == cur:
+                    st[4 * j:4 * j + 4] = h
+                else:
+                    st[4 * j:4 * j + 4] = proof[k + 4 * sib:k + 4 * sib + 4]
+                    sib += 1
+            k += 4 * (arity - 1)
+            h = oracle.permute(st)[:4]
+            pos //= arity
+            n = (n + arity - 1) // arity
+            i += 1
+        assert k == proof.size
+        if llv == 0:
+            assert (h == tree[-4:]).all()
+        else:
+            # the digest reached must be node `pos` of the last verified level
+            off, m = 0, rows
+            while m > lim:
+                m2 = (m + arity - 1) // arity
+                off += m2 * arity
+                m = m2
+            assert (h == tree[(off + pos) * 4:(off + pos) * 4 + 4]).all()
+
+
+# -------------------------------------------------------------------- FRI --
+@pytest.mark.parametrize("prev,cur,nbe", [(10, 7, 12), (9, 7, 9), (8, 5, 10), (6, 5, 8)])
+def test_fri_fold_matches_per_coset_bruteforce(oracle, prev, cur, nbe):
+    pol = datagen.uniform(3 << prev, 1000 + prev)
+    ch = datagen.uniform(3, 5)
+    out = oracle.fri_fold(pol, ch, 1, nbe, prev, cur)
+    n2, nx = 1 << cur, 1 << (prev - cur)
+    for g in [0, 1, 5, n2 - 1]:
+        vals = np.stack([pol[(i * n2 + g) * 3:(i * n2 + g) * 3 + 3] for i in range(nx)])
+        assert (out[g * 3:g * 3 + 3] == oracle.fri_fold_one(vals, ch, nbe, prev, cur, g)).all()
+
+
+def test_fri_fold_is_polynomial_evaluation(oracle):
+    """A base-field polynomial f of degree < 2^prev evaluated on the FRI domain
+    s*<w_{2^prev}> (s = 7^(2^(nbe-prev))) folds to sum_i c_i(g) X^i where the
+    fold of the coset {x_g * w^(i*2^cur)} at challenge X equals
+    sum_k X^k * f_k(x_g^nx) with f = sum_k x^k f_k(x^nx) (even/odd style split)."""
+    prev, cur, nbe = 6, 4, 8
+    nx, n2 = 1 << (prev - cur), 1 << cur
+    rng = random.Random(3)
+    coeffs = [rng.randrange(P) for _ in range(1 << prev)]
+    s = pow(7, 2 ** (nbe - prev), P)
+    w = oracle.root(prev)
+    pol = np.zeros(3 << prev, dtype=np.uint64)
+    for e in range(1 << prev):
+        x = s * pow(w, e, P) % P
+        pol[3 * e] = sum(c * pow(x, k, P) for k, c in enumerate(coeffs)) % P
+    X = [rng.randrange(P), 0, 0]  # base-field challenge keeps the check in F_p
+    out = oracle.fri_fold(pol, np.array(X, dtype=np.uint64), 1, nbe, prev, cur)
+    for g in range(n2):
+        y = pow(s * pow(w, g, P) % P, nx, P)
+        expect = 0
+        for k in range(nx):
+            fk = sum(coeffs[k + nx * m] * pow(y, m, P) for m in range(len(coeffs) // nx)) % P
+            expect = (expect + fk * pow(X[0], k, P)) % P
+        assert int(out[3 * g]) == expect and int(out[3 * g + 1]) == 0
+
+
+def test_fri_transpose(oracle):
+    deg, tb = 64, 3
+    pol = np.arange(3 * deg, dtype=np.uint64)
+    aux = oracle.fri_transpose(pol, deg, tb)
+    w, h = 1 << tb, deg >> tb
+    for i in range(w):
+        for j in range(h):
+            assert (aux[(i * h + j) * 3:(i * h + j) * 3 + 3] == pol[(j * w + i) * 3:(j * w + i) * 3 + 3]).all()
+
+
+# --------------------------------------------------------------- datagen --
+def test_seeded_generator_agrees(oracle):
+    a = oracle.uniform(50000, 0x5EED0001)
+    assert (a == datagen.uniform(50000, 0x5EED0001)).all()
+    assert (a < P).all()
+    assert (datagen.uniform(100, 0x5EED0001, start=7) == a[7:107]).all()
+
+
+# ------------------------------------------------- compiled reference lib --
+def test_compiled_reference_agrees_with_oracle(oracle):
+    from oracle.oracle import RefLib
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref = RefLib()
+    for w in (8, 12, 16):
+        st = datagen.uniform(w, w)
+        assert (ref.permute(st) == oracle.permute(st)).all()
+    for (n, ne, c) in [(16, 32, 8), (256, 1024, 13), (1, 4, 3)]:
+        t = datagen.uniform(n * c, n + c).reshape(n, c)
+        assert (ref.lde(t, ne) == oracle.lde(t, ne)).all()
+    for a in (2, 3, 4):
+        src = datagen.uniform(257 * 9, a).reshape(257, 9)
+        n = oracle.tree_num_elements(257, a)
+        assert (ref.merkletree(src, a, n) == oracle.merkletree(src, a)).all()
+    x, y = datagen.uniform(3, 1), datagen.uniform(3, 2)
+    assert (ref.c3_mul(x, y) == oracle.c3_mul(x, y)).all()
