@@ -114,6 +114,14 @@ int64_t pil2hip_p2_permute(uint64_t *d_states, uint64_t count, uint32_t width, v
  * reference ABI; used to build benchmark inputs directly in HBM. */
 int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_t start, void *stream);
 
+/* Element-wise device field arithmetic, for checking the GPU's Goldilocks
+ * operations against exact integers (no reference counterpart; the reference's
+ * goldilocks_*_ffi entry points, starks_api.hpp:200-214, are the scalar CPU
+ * analogue). op 0: add, 1: sub, 2: mul (canonical results of a[i] op b[i]);
+ * op 3: canonical residue of the 128-bit value b[i]*2^64 + a[i]. */
+int64_t pil2hip_gl_binop(uint64_t *d_out, const uint64_t *d_a, const uint64_t *d_b, uint64_t n, int32_t op,
+                         void *stream);
+
 /* Phase timing of pil2hip_commit with HIP events on the commit's own stream (no
  * reference counterpart; the reference's TimerGPU, utils/gpu_timer.cuh, plays
  * this role). While enabled, every pil2hip_commit synchronises its stream and

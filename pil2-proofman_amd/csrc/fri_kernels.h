@@ -7,3 +7,4 @@ extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_
                                uint64_t prev_bits, uint64_t cur_bits, hipStream_t st);
 extern "C" int fri_launch_transpose(uint64_t *aux, const uint64_t *pol, uint64_t degree, uint32_t t_bits,
                                     hipStream_t st);
+extern "C" int fri_launch_gl_binop(uint64_t *out, const uint64_t *a, const uint64_t *b, uint64_t n, int op, hipStream_t st);

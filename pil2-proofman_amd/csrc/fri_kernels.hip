@@ -135,3 +135,29 @@ extern "C" int fri_launch_transpose(uint64_t *aux, const uint64_t *pol, uint64_t
     hipLaunchKernelGGL(k_fri_transpose, g, dim3(256), 0, st, aux, pol, degree, t_bits);
     return (int)hipGetLastError();
 }
+
+// ------------------------------------------------------- field self-check ----
+// Element-wise field operations on device (op 0 add, 1 sub, 2 mul, 3 canonical
+// reduce of a 128-bit value (a = low word, b = high word)). Exposed so the GPU
+// tests can pin the device arithmetic on edge values against exact integers.
+__global__ __launch_bounds__(256) void k_gl_binop(uint64_t *__restrict__ out, const uint64_t *__restrict__ a,
+                                                  const uint64_t *__restrict__ b, uint64_t n, int op)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t x = a[i], y = b[i];
+    uint64_t r;
+    if (op == 0) r = gl_add(gl_canon(x), gl_canon(y));
+    else if (op == 1) r = gl_sub(gl_canon(x), gl_canon(y));
+    else if (op == 2) r = gl_mul(x, y);
+    else r = gl_reduce128_canon(x, y);
+    out[i] = r;
+}
+
+extern "C" int fri_launch_gl_binop(uint64_t *out, const uint64_t *a, const uint64_t *b, uint64_t n, int op,
+                                   hipStream_t st)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_gl_binop, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, a, b, n, op);
+    return (int)hipGetLastError();
+}

@@ -82,14 +82,16 @@ __device__ __forceinline__ uint64_t pow7(uint64_t x)
 template <int W>
 __device__ __forceinline__ void p2_permute(uint64_t *s)
 {
+    // Round loops stay rolled: the unrolled permutation would not fit the
+    // instruction cache, and per-round constants are wave-uniform scalar loads.
     ext_layer<W>(s);
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
         for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(r * W + i)));
         ext_layer<W>(s);
     }
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < 22; r++) {
         s[0] = pow7(gl_add(s[0], P2Tab<W>::rc(4 * W + r)));
         uint64_t sum = s[0];
@@ -98,7 +100,7 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
 #pragma unroll
         for (int i = 0; i < W; i++) s[i] = gl_add(gl_mul(s[i], P2Tab<W>::d(i)), sum);
     }
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
         for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(4 * W + 22 + r * W + i)));

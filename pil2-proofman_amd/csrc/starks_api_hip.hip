@@ -326,6 +326,15 @@ int64_t pil2hip_p2_permute(uint64_t *d_states, uint64_t count, uint32_t width, v
     return 0;
 }
 
+int64_t pil2hip_gl_binop(uint64_t *d_out, const uint64_t *d_a, const uint64_t *d_b, uint64_t n, int32_t op,
+                         void *stream)
+{
+    if (op < 0 || op > 3) return fail(-26, "op must be 0..3");
+    int e = fri_launch_gl_binop(d_out, d_a, d_b, n, op, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "field op launch failed");
+    return 0;
+}
+
 int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_t start, void *stream)
 {
     int e = ntt_launch_fill_uniform(d_out, n, seed, start, (hipStream_t)stream);

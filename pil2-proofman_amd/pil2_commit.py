@@ -54,6 +54,7 @@ _SIGS = {
     "pil2hip_transpose_cm2rm": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_p2_permute": (_i64, [_vp, _u64, ctypes.c_uint32, _vp]),
     "pil2hip_fill_uniform": (_i64, [_vp, _u64, _u64, _u64, _vp]),
+    "pil2hip_gl_binop": (_i64, [_vp, _vp, _vp, _u64, ctypes.c_int32, _vp]),
     "pil2hip_timing_enable": (None, [ctypes.c_bool]),
     "pil2hip_timing_read": (_u64, [_vp, ctypes.c_bool]),
     "pil2hip_last_error": (ctypes.c_char_p, []),

@@ -38,6 +38,43 @@ def gpu_lde(pc, trace, nb, nbe):
     return colmajor_to_rowmajor(pc.to_host(out), 1 << nbe, c)
 
 
+# ----------------------------------------------------------------- field --
+EDGE = [0, 1, 2, 3, 7, 2**31, 2**32 - 2, 2**32 - 1, 2**32, 2**32 + 1, 2**63 - 1, 2**63, 2**63 + 1,
+        P - 2**32, P - 2, P - 1]
+RAW_EDGE = EDGE + [P, P + 1, P + 2**31, 2**64 - 2, 2**64 - 1]  # non-canonical words (mul accepts any)
+
+
+def _binop(pc, op, a, b):
+    a = np.asarray(a, dtype=np.uint64)
+    b = np.asarray(b, dtype=np.uint64)
+    out = pc.empty(a.size)
+    da, db = pc.to_device(a), pc.to_device(b)
+    rc = pc.lib.pil2hip_gl_binop(pc._ptr(out), pc._ptr(da), pc._ptr(db), a.size, op, pc._stream())
+    assert rc == 0
+    _sync()
+    return [int(v) for v in pc.to_host(out)]
+
+
+def _pairs(vals, rng, n_random):
+    a = [x for x in vals for _ in vals] + [int(v) for v in rng.integers(0, 2**64, n_random, dtype=np.uint64)]
+    b = [y for _ in vals for y in vals] + [int(v) for v in rng.integers(0, 2**64, n_random, dtype=np.uint64)]
+    return a, b
+
+
+def test_field_ops_edge_values(pc):
+    """Device add/sub/mul/reduce against exact integer arithmetic on all pairs of
+    edge words (carries, borrows, values near p and 2^64) plus random words."""
+    rng = np.random.default_rng(11)
+    a, b = _pairs(RAW_EDGE, rng, 20000)
+    assert _binop(pc, 2, a, b) == [(x * y) % P for x, y in zip(a, b)]
+    assert _binop(pc, 0, a, b) == [(x + y) % P for x, y in zip(a, b)]
+    assert _binop(pc, 1, a, b) == [(x - y) % P for x, y in zip(a, b)]
+    # 128-bit reduction with words chosen to hit every carry/borrow combination
+    words = [0, 1, 2**32 - 1, 2**32, 2**63, 2**64 - 2**32, 2**64 - 2**32 + 1, 2**64 - 1]
+    lo, hi = _pairs(words, rng, 20000)
+    assert _binop(pc, 3, lo, hi) == [((h << 64) + l) % P for l, h in zip(lo, hi)]
+
+
 # ------------------------------------------------------------- Poseidon2 --
 @pytest.mark.parametrize("w", [8, 12, 16])
 def test_permute_reference_goldens(pc, refvec, w):
