@@ -116,7 +116,7 @@ int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_
 
 /* Element-wise device field arithmetic, for checking the GPU's Goldilocks
  * operations against exact integers (no reference counterpart; the reference's
- * goldilocks_*_ffi entry points, starks_api.hpp:200-214, are the scalar CPU
+ * goldilocks_*_ffi entry points, starks_api.hpp:172-185, are the scalar CPU
  * analogue). op 0: add, 1: sub, 2: mul (canonical results of a[i] op b[i]);
  * op 3: canonical residue of the 128-bit value b[i]*2^64 + a[i]. */
 int64_t pil2hip_gl_binop(uint64_t *d_out, const uint64_t *d_a, const uint64_t *d_b, uint64_t n, int32_t op,
