@@ -55,30 +55,34 @@ def host_info():
     return info
 
 
-def cpu_baseline(cfg, budget_cols=2):
+def cpu_baseline(cfg, target_s=15.0, max_cols=None):
     """Time the reference's own CPU path (oracle/_ref: NTT_Goldilocks::LDE +
     Poseidon2Goldilocks<W>::merkletree, OpenMP over all host cores) on a bounded
-    sample of the same workload: full trace height, same blowup and arity,
-    `budget_cols` columns. Falls back to our C restatement if the reference
-    library cannot be loaded here."""
-    import numpy as np
+    sample of the same workload: full trace height, same blowup and arity, and as
+    many columns as fit about `target_s` seconds (sized from a 1-column probe).
+    Falls back to our C restatement if the reference library cannot be loaded."""
     import datagen
     from oracle.oracle import Oracle, RefLib
     n, ne, arity = 1 << cfg["n_bits"], 1 << (cfg["n_bits"] + cfg["blowup_bits"]), cfg["arity"]
-    cols = budget_cols
-    trace = datagen.uniform(n * cols, cfg["seed"]).reshape(n, cols)
     o = Oracle()
     try:
         lib, kind = RefLib(), "reference"
         tree_len = o.tree_num_elements(ne, arity)
-        run = lambda: lib.merkletree(lib.lde(trace, ne), arity, tree_len)
+        run = lambda t: lib.merkletree(lib.lde(t, ne), arity, tree_len)
         threads = lib.num_threads()
     except (OSError, FileNotFoundError):
         lib, kind = o, "port"
-        run = lambda: o.merkletree(o.lde(trace, ne), arity)
+        run = lambda t: o.merkletree(o.lde(t, ne), arity)
         threads = o.num_threads()
+    pc_ = 4
+    probe = datagen.uniform(n * pc_, cfg["seed"]).reshape(n, pc_)
     t0 = time.perf_counter()
-    run()
+    run(probe)
+    t_probe = time.perf_counter() - t0
+    cols = int(max(1, min(max_cols or cfg["cols"], pc_ * target_s / max(t_probe, 1e-3))))
+    trace = datagen.uniform(n * cols, cfg["seed"]).reshape(n, cols)
+    t0 = time.perf_counter()
+    run(trace)
     dt = time.perf_counter() - t0
     return {"value": ne * cols / dt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
             "sample": "2^%d x %d cols -> 2^%d rows (blowup %d), Poseidon2 W%d arity %d; %.2f s"
@@ -87,15 +91,45 @@ def cpu_baseline(cfg, budget_cols=2):
             **host_info()}
 
 
+def _profile_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes):
+    """Plain device-to-device copy of a known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="plain_large", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate duration of the bounded CPU-baseline sample")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the commit path shards per job / per column group, see DESIGN.md)", file=sys.stderr)
@@ -145,14 +179,22 @@ def main():
     leaf_s = phase_ms[4] / 1e3
     leaf_bytes = 8 * ne * cols + 32 * ne  # reads every LDE word once, writes one digest per row
     muls_per_s = perms * MULS_PER_PERM[width] / leaf_s
+    # VALU roofline of the leaf kernel: algorithmic Goldilocks multiplications
+    # (736 per W12 permutation) against the rate at which the VALU can issue the
+    # four 32x32->64 partial products each one needs (v_mad_u64_u32 at the
+    # single-issue lane rate), i.e. a products-only ceiling.
+    mul_peak = VALU_PEAK_TOPS / 4.0  # T mul/s
+    traffic = _profile_traffic("k_p2_leaves<%d>" % width, args.config)
     roofline = {
         "bound": "valu", "kernel": "k_p2_leaves<%d>" % width,
-        "achieved": round(perms / leaf_s / 1e9, 4), "unit": "Gperm/s",
-        "peak": None, "frac": None,
-        "field_muls_per_s": muls_per_s,
-        "valu_peak_lane_ops_per_s": VALU_PEAK_TOPS * 1e12,
+        "achieved": round(muls_per_s / 1e12, 4), "peak": round(mul_peak, 4), "unit": "Tmul/s",
+        "frac": round(muls_per_s / 1e12 / mul_peak, 4),
+        "model": "64x64 field products: %d per W%d permutation, 4 v_mad_u64_u32 each at %.1f T lane-op/s "
+                 "(published 157.3 TFLOP/s FP32 vector / 2 FMA / 2 packed)" % (MULS_PER_PERM[width], width,
+                                                                             VALU_PEAK_TOPS),
+        "perm_per_s": perms / leaf_s,
         "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": traffic,
         "avg_ms": phase_ms[4],
     }
     # LDE passes: algorithmic bytes 8*C*(N + NExt) over the four LDE kernels
@@ -183,7 +225,7 @@ def main():
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_cols)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -189,6 +189,19 @@ __host__ __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b)
 #endif
 }
 
+// any 64-bit a, b -> 64-bit word congruent to a * b (not necessarily canonical);
+// for products that only feed further multiplications
+__host__ __device__ __forceinline__ uint64_t gl_mul_nc(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t x0, x1, x2, x3;
+    gl_mul_words(a, b, x0, x1, x2, x3);
+    return gl_reduce_words(x0, x1, x2, x3);
+#else
+    return gl_mul(a, b);
+#endif
+}
+
 // canonical value of the 128-bit number (hi:lo)
 __host__ __device__ __forceinline__ uint64_t gl_reduce128_canon(uint64_t lo, uint64_t hi)
 {

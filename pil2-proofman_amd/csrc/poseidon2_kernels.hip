@@ -71,11 +71,12 @@ __device__ __forceinline__ void ext_layer(uint64_t *s)
     for (int i = 0; i < W; i++) s[i] = gl_add(s[i], c[i & 3]);
 }
 
+// x^7; intermediate powers only feed multiplications, so they stay non-canonical
 __device__ __forceinline__ uint64_t pow7(uint64_t x)
 {
-    uint64_t x2 = gl_mul(x, x);
-    uint64_t x3 = gl_mul(x2, x);
-    uint64_t x4 = gl_mul(x2, x2);
+    uint64_t x2 = gl_mul_nc(x, x);
+    uint64_t x3 = gl_mul_nc(x2, x);
+    uint64_t x4 = gl_mul_nc(x2, x2);
     return gl_mul(x3, x4);
 }
 
