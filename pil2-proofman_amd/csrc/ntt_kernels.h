@@ -3,17 +3,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define NTT_THREADS 512
-#define NTT_LOG_LDS 13                     // 8192 u64 = 64 KiB of LDS per block
-#define NTT_LDS_ELEMS (1u << NTT_LOG_LDS)
+#define NTT_THREADS 1024                   // max threads per block (16 values each)
+#define NTT_LOG_TILE 14                    // 2^14 values per block tile
+#define NTT_LDS_WORDS (16384 + 1024)       // tile + one pad word per 16 (139,264 bytes)
 #define NTT_MAX_LOG 24                     // largest transform handled (2^24 rows)
 
 struct NttTablesHost {
-    uint64_t *roots_f, *roots_i;           // NTT_LDS_ELEMS entries each
+    uint64_t *t14f, *t14i;                 // 2^14 entries each: w_{2^14}^(+-y)
     uint64_t *twl_f, *twh_f, *twl_i, *twh_i;  // 4096 entries each
     uint64_t *shl, *shh;                   // 4096 entries each
 };
 
+extern "C" int ntt_init_device(void);
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
                               uint64_t ncols, const NttTablesHost *th, hipStream_t st, hipEvent_t *ev);
 extern "C" int ntt_launch_cm2rm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_cs,

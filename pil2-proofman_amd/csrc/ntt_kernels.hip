@@ -9,35 +9,40 @@
 // is produced column-major (column c at lde + c*NExt), which keeps every NTT pass
 // and the row hashing coalesced.
 //
-// Transform plan (four-step, no explicit bit-reversal pass):
-//   iNTT (size N = n1*n2)   pass 1: strided DIF over m (n1 points, stride n2),
-//                                   then x w_N^-(i*bitrev(m))
-//                           pass 2: contiguous DIF over i (n2 points) -> the
-//                                   coefficients in bit-reversed order
-//   coset+extend            scale coefficient e by 7^e/N; in bit-reversed order
-//                           the zero-padded array of size NExt = N*B holds each
-//                           coefficient followed by B-1 zeros, and the first
-//                           log2(B) DIT stages turn that into B copies
-//   NTT (size NExt = n1*n2B) pass 3: contiguous DIT over blocks of n2*B
-//                                   (fused with pass 2 in one kernel), then
-//                                   x w_NExt^(bitrev(m)*k)
-//                           pass 4: strided DIT over m (n1 points, stride n2*B)
-//                                   -> natural order
+// Transform plan (four-step, no explicit bit-reversal pass), N = n1*n2, NExt = N*B,
+// with n2*B <= 2^14 (one LDS tile) and n1 <= 2^10:
+//   pass 1  strided iNTT: n1-point DIF over m (stride n2), then x w_N^-(i*bitrev(m))
+//   pass 2  one block per (column, m): n2-point DIF -> coefficients in bit-reversed
+//           order, x 7^e/N, spread (in bit-reversed order the zero-padded array
+//           holds each coefficient followed by B-1 zeros, and the first log2(B) DIT
+//           stages make B copies of it), n2*B-point DIT from stage log2(B) on,
+//           then x w_NExt^(bitrev(m)*k)
+//   pass 3  strided NTT: n1-point DIT over m (stride n2*B) -> natural order
+//
+// Inside a tile every sub-transform of 2^L points runs as rounds of at most 4
+// index bits (four-step again): a 2^r-point DFT in registers whose internal
+// twiddles are powers of w_16, and one twiddle per element between rounds,
+// w_{2^(q+r)}^(o*bitrev(j)) (o = the element's index bits below the round).
+// Each thread keeps 16 values per round; rounds exchange through LDS. The first
+// round of a pass reads global memory and the last one writes it, both in runs
+// of >= 128 bytes.
 #include "gl_field.h"
 #include "ntt_kernels.h"
 
 // --------------------------------------------------------------- tables ----
-// All twiddles derive from the reference's root chain W[k] (goldilocks_base_field.cpp:11-44).
-//   roots_f[h + j] = w_{2h}^j        (h = 1, 2, ..., LDS_ELEMS/2; j < h)   sub-FFT stages
-//   roots_i[h + j] = w_{2h}^-j
-//   tw_f[0][x & 4095] * tw_f[1][x >> 12] = w_{2^24}^x                     inter-pass
-//   sh[0][e & 4095] * sh[1][e >> 12] = 7^e                                  coset shift
+// All roots derive from the reference's chain W[k] (goldilocks_base_field.cpp:11-44).
+//   t14f[y] = w_{2^14}^y, t14i[y] = w_{2^14}^-y        (y < 2^14)    inside tiles
+//   twl/twh: w_{2^24}^x = lo[x & 4095] * hi[x >> 12]   (x < 2^24)    between passes
+//   shl/shh: 7^e = lo[e & 4095] * hi[e >> 12]                        coset shift
 struct NttTables {
-    const uint64_t *roots_f;
-    const uint64_t *roots_i;
+    const uint64_t *t14f, *t14i;
     const uint64_t *twl_f, *twh_f, *twl_i, *twh_i;
     const uint64_t *shl, *shh;
 };
+
+// w_16^k and w_16^-k for k < 8 (w_16 = W[4] = 2^12)
+__constant__ uint64_t c_w16f[8];
+__constant__ uint64_t c_w16i[8];
 
 __device__ __forceinline__ uint64_t tw_pow(const uint64_t *lo, const uint64_t *hi, uint32_t x)
 {
@@ -45,6 +50,8 @@ __device__ __forceinline__ uint64_t tw_pow(const uint64_t *lo, const uint64_t *h
     uint32_t h = x >> 12;
     return h ? gl_mul(a, hi[h]) : a;
 }
+
+__device__ __forceinline__ uint32_t lds_addr(uint32_t x) { return x + (x >> 4); }
 
 // ------------------------------------------------------------ transpose ----
 // in: rows x cols row-major (row stride in_rs), out: column c at out + c*out_cs.
@@ -84,187 +91,479 @@ __global__ __launch_bounds__(256) void k_transpose_cm2rm(const uint64_t *__restr
     }
 }
 
-// ---------------------------------------------------------- LDS stages ----
-// Sub-FFT over "m" inside an LDS tile laid out [m][ti] (ti fastest), n points,
-// tcount independent lines. DIF: natural in -> bit-reversed out. DIT: reverse.
-template <bool INV>
-__device__ __forceinline__ void lds_dif(uint64_t *s, uint32_t logn, uint32_t tcount, uint32_t log_t,
-                                        const uint64_t *roots)
+// ------------------------------------------------------ small DFTs -------
+// R-point DIF (natural in -> bit-reversed out) on v[0..R), roots w_R^k = w_16^(k*16/R).
+template <int R> struct Log2 { static constexpr int v = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1; };
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_dif(uint64_t *v)
 {
-    uint32_t n = 1u << logn;
-    uint32_t nbf = (n >> 1) << log_t;
-    for (int t = (int)logn - 1; t >= 0; t--) {
-        uint32_t h = 1u << t;
-        for (uint32_t b = threadIdx.x; b < nbf; b += blockDim.x) {
-            uint32_t ii = b & (tcount - 1), j = b >> log_t;
-            uint32_t jj = j & (h - 1);
-            uint32_t p0 = ((j >> t) << (t + 1)) + jj;
-            uint32_t a0 = (p0 << log_t) + ii, a1 = ((p0 + h) << log_t) + ii;
-            uint64_t u = s[a0], v = s[a1];
-            s[a0] = gl_add(u, v);
-            uint64_t d = gl_sub(u, v);
-            s[a1] = jj ? gl_mul(d, roots[h + jj]) : d;
+    // stage loops use a linear counter so they unroll completely (a shifted
+    // counter left v[] dynamically indexed)
+#pragma unroll
+    for (int st = Log2<R>::v - 1; st >= 0; st--) {
+        const int h = 1 << st;
+#pragma unroll
+        for (int b = 0; b < R; b += 2 * h) {
+#pragma unroll
+            for (int jj = 0; jj < h; jj++) {
+                uint64_t u = v[b + jj], x = v[b + jj + h];
+                v[b + jj] = gl_add(u, x);
+                uint64_t d = gl_sub(u, x);
+                // w_{2h}^jj = w_16^(jj * 8 / h)
+                v[b + jj + h] = jj ? gl_mul(d, INV ? c_w16i[jj * 8 / h] : c_w16f[jj * 8 / h]) : d;
+            }
         }
-        __syncthreads();
     }
 }
 
-__device__ __forceinline__ void lds_dit(uint64_t *s, uint32_t first_t, uint32_t logn, uint32_t tcount, uint32_t log_t,
-                                        const uint64_t *roots)
+// R-point DIT (bit-reversed in -> natural out)
+template <int R, bool INV>
+__device__ __forceinline__ void dft_dit(uint64_t *v)
 {
-    uint32_t n = 1u << logn;
-    uint32_t nbf = (n >> 1) << log_t;
-    for (uint32_t t = first_t; t < logn; t++) {
-        uint32_t h = 1u << t;
-        for (uint32_t b = threadIdx.x; b < nbf; b += blockDim.x) {
-            uint32_t ii = b & (tcount - 1), j = b >> log_t;
-            uint32_t jj = j & (h - 1);
-            uint32_t p0 = ((j >> t) << (t + 1)) + jj;
-            uint32_t a0 = (p0 << log_t) + ii, a1 = ((p0 + h) << log_t) + ii;
-            uint64_t u = s[a0], v = s[a1];
-            if (jj) v = gl_mul(v, roots[h + jj]);
-            s[a0] = gl_add(u, v);
-            s[a1] = gl_sub(u, v);
+#pragma unroll
+    for (int st = 0; st < Log2<R>::v; st++) {
+        const int h = 1 << st;
+#pragma unroll
+        for (int b = 0; b < R; b += 2 * h) {
+#pragma unroll
+            for (int jj = 0; jj < h; jj++) {
+                uint64_t u = v[b + jj], x = v[b + jj + h];
+                if (jj) x = gl_mul(x, INV ? c_w16i[jj * 8 / h] : c_w16f[jj * 8 / h]);
+                v[b + jj] = gl_add(u, x);
+                v[b + jj + h] = gl_sub(u, x);
+            }
         }
-        __syncthreads();
     }
 }
 
-// -------------------------------------------------------- strided passes ----
-// Column c = blockIdx.y at data + c*cs; length n = n1 * n2 (log_n = log_n1 + log_n2).
-// A block owns the n1 points {m*n2 + i : m} for ti consecutive i (i0 .. i0+ti).
-// INV = true : DIF with inverse roots, then x w_n^-(i * bitrev(m))  (iNTT pass 1)
-// INV = false: DIT with forward roots on bit-reversed m, no twiddle (NTT pass 4)
-template <bool INV>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided(uint64_t *__restrict__ data, uint64_t cs, uint32_t log_n1,
-                                                             uint32_t log_n2, uint32_t log_t, NttTables tb)
+// Element position of value j of group gidx in a round over index bits [q, q+r)
+// of a tile with 2^lt lines (line index fastest). Returns the tile-linear index
+// (m << lt) | ii and the pieces the kernels need.
+struct RoundPos {
+    uint32_t ii, o_lo, o_hi;
+};
+__device__ __forceinline__ RoundPos round_pos(uint32_t gidx, uint32_t lt, uint32_t q)
+{
+    RoundPos p;
+    p.ii = gidx & ((1u << lt) - 1);
+    uint32_t rest = gidx >> lt;
+    p.o_lo = rest & ((1u << q) - 1);
+    p.o_hi = rest >> q;
+    return p;
+}
+__device__ __forceinline__ uint32_t round_m(const RoundPos &p, uint32_t j, uint32_t q, uint32_t r)
+{
+    return (p.o_hi << (q + r)) | (j << q) | p.o_lo;
+}
+
+// --------------------------------------------------- strided passes ------
+// Tile: n1 = 2^L points (index m, global stride n2) x 2^lt lines (index ii, global
+// offset i0 + ii); column c = blockIdx.y at data + c*cs. The tile holds 2^(L+lt)
+// values, 16 per thread, in LDS at tile index x = (m << lt) | ii. A load phase
+// and a store phase move the tile in runs of 2^lt >= 16 consecutive words; the
+// rounds between them work in LDS only.
+//
+// DIF rounds (iNTT pass 1): index bits from the top, 2^r-point DIF, then value j
+// times w_{2^(q+r)}^-(o_lo * bitrev(j)).
+template <int R>
+__device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint32_t lt, uint32_t q,
+                                                  const NttTables &tb)
+{
+    constexpr int LR = Log2<R>::v;
+    constexpr int G = 16 / R;
+    uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        RoundPos p = round_pos(t * G + gg, lt, q);
+        uint64_t v[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
+        dft_dif<R, true>(v);
+        if (p.o_lo) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
+                v[j] = gl_mul(v[j], tb.t14i[y]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
+    }
+}
+
+// DIT rounds (NTT pass 3): index bits from the bottom; value j times
+// w_{2^(q+r)}^(bitrev(j) * o_lo), then the 2^r-point DIT.
+template <int R>
+__device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint32_t lt, uint32_t q,
+                                                  const NttTables &tb)
+{
+    constexpr int LR = Log2<R>::v;
+    constexpr int G = 16 / R;
+    uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        RoundPos p = round_pos(t * G + gg, lt, q);
+        uint64_t v[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
+        if (p.o_lo) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
+                v[j] = gl_mul(v[j], tb.t14f[y]);
+            }
+        }
+        dft_dit<R, false>(v);
+#pragma unroll
+        for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
+    }
+}
+
+__device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ col, uint64_t n2,
+                                             uint64_t i0, uint32_t lt, uint32_t tile)
+{
+    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
+        s[lds_addr(x)] = gl_canon(col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))]);
+    __syncthreads();
+}
+
+// iNTT pass 1: after the DIF the value at m is the transform at bitrev(m); it is
+// stored times w_N^-(i * bitrev_L(m)) (i = i0 + ii, N = n1 * n2).
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
+                                                                 uint32_t log_n2, uint32_t lt, NttTables tb)
 {
     extern __shared__ uint64_t s[];
-    uint32_t ti = 1u << log_t, n1 = 1u << log_n1;
     uint64_t n2 = 1ull << log_n2;
-    uint64_t i0 = (uint64_t)blockIdx.x << log_t;
+    uint64_t i0 = (uint64_t)blockIdx.x << lt;
     uint64_t *col = data + (uint64_t)blockIdx.y * cs;
-    uint32_t tile = n1 << log_t;
+    uint32_t tile = 1u << (L + lt);
+    strided_load(s, col, n2, i0, lt, tile);
+    int q = (int)L;
+    while (q > 0) {
+        int r = q >= 4 ? 4 : q;
+        q -= r;
+        switch (r) {
+        case 4: strided_dif_round<16>(s, lt, q, tb); break;
+        case 3: strided_dif_round<8>(s, lt, q, tb); break;
+        case 2: strided_dif_round<4>(s, lt, q, tb); break;
+        default: strided_dif_round<2>(s, lt, q, tb); break;
+        }
+        __syncthreads();
+    }
+    uint32_t shift_n = 24 - (L + log_n2);
     for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
-        uint32_t m = x >> log_t, ii = x & (ti - 1);
-        s[x] = col[(uint64_t)m * n2 + i0 + ii];
+        uint32_t m = x >> lt, ii = x & ((1u << lt) - 1);
+        uint32_t e = (uint32_t)((i0 + ii) * bitrev32(m, L));
+        uint64_t v = s[lds_addr(x)];
+        if (e) v = gl_mul(v, tw_pow(tb.twl_i, tb.twh_i, e << shift_n));
+        col[(uint64_t)m * n2 + i0 + ii] = v;
     }
-    __syncthreads();
-    if (INV) {
-        lds_dif<true>(s, log_n1, ti, log_t, tb.roots_i);
-        uint32_t shift = 24 - (log_n1 + log_n2);  // w_n = w_{2^24}^(2^shift)
-        for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
-            uint32_t m = x >> log_t, ii = x & (ti - 1);
-            uint32_t e = (uint32_t)((i0 + ii) * bitrev32(m, log_n1));
-            uint64_t v = s[x];
-            if (e) v = gl_mul(v, tw_pow(tb.twl_i, tb.twh_i, e << shift));
-            col[(uint64_t)m * n2 + i0 + ii] = v;
+}
+
+// NTT pass 3: n1-point DIT over m (bit-reversed in, natural out).
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
+                                                                 uint32_t log_n2, uint32_t lt, NttTables tb)
+{
+    extern __shared__ uint64_t s[];
+    uint64_t n2 = 1ull << log_n2;
+    uint64_t i0 = (uint64_t)blockIdx.x << lt;
+    uint64_t *col = data + (uint64_t)blockIdx.y * cs;
+    uint32_t tile = 1u << (L + lt);
+    strided_load(s, col, n2, i0, lt, tile);
+    uint32_t q = 0;
+    while (q < L) {
+        uint32_t r = (L - q) >= 4 ? 4 : (L - q);
+        switch (r) {
+        case 4: strided_dit_round<16>(s, lt, q, tb); break;
+        case 3: strided_dit_round<8>(s, lt, q, tb); break;
+        case 2: strided_dit_round<4>(s, lt, q, tb); break;
+        default: strided_dit_round<2>(s, lt, q, tb); break;
         }
-    } else {
-        lds_dit(s, 0, log_n1, ti, log_t, tb.roots_f);
-        for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
-            uint32_t m = x >> log_t, ii = x & (ti - 1);
-            col[(uint64_t)m * n2 + i0 + ii] = s[x];
-        }
+        __syncthreads();
+        q += r;
     }
+    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
+        col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
 }
 
 // ------------------------------------------------------ fused LDE block ----
-// Block (m = blockIdx.x, column c = blockIdx.y):
-//   load n2 values of the half-transformed column (scratch, column stride cs_in)
-//   DIF over i with inverse roots -> coefficients in bit-reversed order,
-//   scale coefficient e = bitrev_L(m*n2 + q) by 7^e / N,
-//   write each into B consecutive slots, DIT stages log2(B) .. log2(n2*B)-1,
-//   multiply by w_NExt^(bitrev_{log n1}(m) * k) and store block m of the
-//   column-major LDE (column stride cs_out).
-__global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
-                                                           uint64_t *__restrict__ out, uint64_t cs_out,
-                                                           uint32_t log_n1, uint32_t log_n2, uint32_t log_b,
-                                                           uint64_t n_inv, NttTables tb)
+// Block (m = blockIdx.x, column c = blockIdx.y); line of n2 = 2^L2 values, block
+// of n2*B/16 threads.
+// Phase 1: the n2 values are loaded into LDS, then DIF rounds run with V = 16/B
+// values per thread (so every thread works); the last round scales the
+// coefficient at position k (index e = bitrev_{L1+L2}(m*n2 + k)) by 7^e / N.
+template <int R, int V>
+__device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32_t q, bool last, uint32_t m,
+                                                uint32_t L2, uint32_t log_n, uint64_t n_inv, const NttTables &tb)
 {
-    extern __shared__ uint64_t s[];
-    uint32_t n2 = 1u << log_n2, nb = n2 << log_b, m = blockIdx.x;
-    const uint64_t *src = in + (uint64_t)blockIdx.y * cs_in + ((uint64_t)m << log_n2);
-    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x) s[q] = src[q];
-    __syncthreads();
-    lds_dif<true>(s, log_n2, 1, 0, tb.roots_i);
-    uint32_t log_n = log_n1 + log_n2;
-    // scale and spread: thread-private registers hold the values between syncs
-    uint64_t v[NTT_LDS_ELEMS / NTT_THREADS];
-    uint32_t k = 0;
-    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x, k++) {
-        uint32_t e = bitrev32((m << log_n2) + q, log_n);
-        v[k] = gl_mul(gl_mul(s[q], n_inv), tw_pow(tb.shl, tb.shh, e));
-    }
-    __syncthreads();
-    k = 0;
-    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x, k++)
-        for (uint32_t r = 0; r < (1u << log_b); r++) s[(q << log_b) + r] = v[k];
-    __syncthreads();
-    lds_dit(s, log_b, log_n2 + log_b, 1, 0, tb.roots_f);
-    uint64_t *dst = out + (uint64_t)blockIdx.y * cs_out + ((uint64_t)m << (log_n2 + log_b));
-    if (log_n1 == 0) {
-        for (uint32_t x = threadIdx.x; x < nb; x += blockDim.x) dst[x] = s[x];
-    } else {
-        uint32_t e1 = bitrev32(m, log_n1);
-        uint32_t shift = 24 - (log_n + log_b);
-        for (uint32_t x = threadIdx.x; x < nb; x += blockDim.x) {
-            uint32_t e = e1 * x;
-            uint64_t val = s[x];
-            if (e) val = gl_mul(val, tw_pow(tb.twl_f, tb.twh_f, e << shift));
-            dst[x] = val;
+    constexpr int LR = Log2<R>::v;
+    constexpr int G = V / R;
+    uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        RoundPos p = round_pos(t * G + gg, 0, q);
+        uint64_t v[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) v[j] = s[lds_addr(round_m(p, j, q, LR))];
+        dft_dif<R, true>(v);
+        if (p.o_lo) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
+                v[j] = gl_mul(v[j], tb.t14i[y]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            uint32_t k = round_m(p, j, q, LR);
+            uint64_t x = v[j];
+            if (last) {
+                uint32_t e = bitrev32((m << L2) | k, log_n);
+                x = gl_mul(gl_mul(x, n_inv), tw_pow(tb.shl, tb.shh, e));
+            }
+            s[lds_addr(k)] = x;
         }
     }
 }
 
+template <int V>
+__device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t m, uint32_t L2, uint32_t log_n,
+                                             uint64_t n_inv, const NttTables &tb)
+{
+    constexpr int LV = Log2<V>::v;
+    bool active = threadIdx.x < ((1u << L2) >> LV);
+    int q = (int)L2;
+    while (q > 0) {
+        int r = q >= LV ? LV : q;
+        q -= r;
+        bool last = (q == 0);
+        if (active) {
+            switch (r) {
+            case 4: if constexpr (V >= 16) block_dif_round<16, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            case 3: if constexpr (V >= 8) block_dif_round<8, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            case 2: if constexpr (V >= 4) block_dif_round<4, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            default: block_dif_round<2, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Phase 2: DIT rounds over bits [logb, logb + L2) of the spread array of n2*B
+// values. The first round reads coefficient (o_hi << r) | j for every copy
+// o_lo < B (the spread), the last one applies w_NExt^(bitrev_L1(m) * P) and
+// stores position P of block m.
+template <int R>
+__device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64_t *__restrict__ dst, uint32_t q,
+                                                uint32_t logb, bool first, bool last, uint32_t e1, uint32_t shift_e,
+                                                const NttTables &tb)
+{
+    constexpr int LR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
+    constexpr int G = 16 / R;
+    uint32_t t = threadIdx.x;
+    uint64_t v[G][R];
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        RoundPos p = round_pos(t * G + gg, 0, q);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            uint32_t P = round_m(p, j, q, LR);
+            v[gg][j] = first ? s[lds_addr(P >> logb)] : s[lds_addr(P)];
+        }
+    }
+    if (first) __syncthreads();  // the spread overwrites coefficients other threads still read
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        RoundPos p = round_pos(t * G + gg, 0, q);
+        if (p.o_lo) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
+                v[gg][j] = gl_mul(v[gg][j], tb.t14f[y]);
+            }
+        }
+        dft_dit<R, false>(v[gg]);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            uint32_t P = round_m(p, j, q, LR);
+            uint64_t x = v[gg][j];
+            if (last) {
+                uint32_t e = e1 * P;
+                if (e) x = gl_mul(x, tw_pow(tb.twl_f, tb.twh_f, e << shift_e));
+                dst[P] = x;
+            } else {
+                s[lds_addr(P)] = x;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
+                                                           uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
+                                                           uint32_t L2, uint32_t logb, uint64_t n_inv, NttTables tb)
+{
+    extern __shared__ uint64_t s[];
+    uint32_t m = blockIdx.x;
+    const uint64_t *src = in + (uint64_t)blockIdx.y * cs_in + ((uint64_t)m << L2);
+    uint32_t log_n = L1 + L2;
+    // phase 1: load the line, iNTT DIF over L2 bits with 16/B values per thread
+    for (uint32_t k = threadIdx.x; k < (1u << L2); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
+    __syncthreads();
+    switch (logb) {
+    case 0: block_phase1<16>(s, m, L2, log_n, n_inv, tb); break;
+    case 1: block_phase1<8>(s, m, L2, log_n, n_inv, tb); break;
+    case 2: block_phase1<4>(s, m, L2, log_n, n_inv, tb); break;
+    default: block_phase1<2>(s, m, L2, log_n, n_inv, tb); break;
+    }
+    bool first = true;
+    // phase 2: NTT DIT over bits [logb, logb + L2) of the n2*B spread values
+    uint64_t *dst = out + (uint64_t)blockIdx.y * cs_out + ((uint64_t)m << (L2 + logb));
+    uint32_t e1 = bitrev32(m, L1);
+    uint32_t shift_e = 24 - (log_n + logb);
+    uint32_t top = L2 + logb;
+    uint32_t qq = logb;
+    first = true;
+    while (qq < top) {
+        uint32_t r = (top - qq) >= 4 ? 4 : (top - qq);
+        bool last = (qq + r == top);
+        switch (r) {
+        case 4: block_dit_round<16>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 3: block_dit_round<8>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 2: block_dit_round<4>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        default: block_dit_round<2>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        }
+        if (!last) __syncthreads();
+        first = false;
+        qq += r;
+    }
+}
+
+// --------------------------------------------------------- tiny LDE -------
+// N < 16 (a column shorter than one thread's group): one lane per output row j
+// of one column evaluates sum_k a_k (7 w_NExt^j)^k after a direct O(N^2) iNTT.
+__global__ __launch_bounds__(256) void k_lde_tiny(const uint64_t *__restrict__ in, uint64_t cs_in,
+                                                  uint64_t *__restrict__ out, uint64_t cs_out, uint32_t log_n,
+                                                  uint32_t log_ext, uint64_t n_inv, uint64_t w_n_inv, uint64_t w_ext)
+{
+    uint32_t n = 1u << log_n, ne = 1u << log_ext;
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    const uint64_t *x = in + (uint64_t)blockIdx.y * cs_in;
+    uint64_t pt = gl_mul(7, gl_pow(w_ext, j)), acc = 0, xp = 1;
+    for (uint32_t k = 0; k < n; k++) {
+        uint64_t a = 0, wk = gl_pow(w_n_inv, k), wp = 1;  // a_k = (1/N) sum_i x_i w_N^-ik
+        for (uint32_t i = 0; i < n; i++) {
+            a = gl_add(a, gl_mul(gl_canon(x[i]), wp));
+            wp = gl_mul(wp, wk);
+        }
+        acc = gl_add(acc, gl_mul(gl_mul(a, n_inv), xp));
+        xp = gl_mul(xp, pt);
+    }
+    out[(uint64_t)blockIdx.y * cs_out + j] = acc;
+}
+
 // ------------------------------------------------------------ launchers ----
+static const uint64_t kW[33] = {
+    0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
+    2198989700608ULL, 4404853092538523347ULL, 6434636298004421797ULL, 4255134452441852017ULL,
+    9113133275150391358ULL, 4355325209153869931ULL, 4308460244895131701ULL,
+    7126024226993609386ULL, 1873558160482552414ULL, 8167150655112846419ULL,
+    5718075921287398682ULL, 3411401055030829696ULL, 8982441859486529725ULL,
+    1971462654193939361ULL, 6553637399136210105ULL, 8124823329697072476ULL,
+    5936499541590631774ULL, 2709866199236980323ULL, 8877499657461974390ULL,
+    3757607247483852735ULL, 4969973714567017225ULL, 2147253751702802259ULL,
+    2530564950562219707ULL, 1905180297017055339ULL, 3524815499551269279ULL,
+    7277203076849721926ULL};  // goldilocks_base_field.cpp:11-44
+
+static bool g_attr_done = false;
+
+extern "C" int ntt_init_device(void)
+{
+    // inverse 16th roots for the register DFTs, and > 64 KiB of dynamic LDS
+    uint64_t w16f[8], w16i[8];
+    uint64_t wi = gl_inv(kW[4]), a = 1, b = 1;
+    for (int k = 0; k < 8; k++) {
+        w16f[k] = a;
+        w16i[k] = b;
+        a = gl_mul(a, kW[4]);
+        b = gl_mul(b, wi);
+    }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_w16f), w16f, sizeof(w16f));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_w16i), w16i, sizeof(w16i));
+    if (e != hipSuccess) return (int)e;
+    if (!g_attr_done) {
+        int bytes = (int)(NTT_LDS_WORDS * 8);
+        e = hipFuncSetAttribute((const void *)k_ntt_strided_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_lde_block, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return (int)e;
+        g_attr_done = true;
+    }
+    return 0;
+}
+
+static size_t lds_bytes(uint32_t log_tile) { return (size_t)8 * ((1u << log_tile) + (1u << log_tile) / 16); }
+
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
                               uint64_t ncols, const NttTablesHost *th, hipStream_t st, hipEvent_t *ev)
 {
-    // ev (optional): events recorded after passes 1..4 (ev[0..3]) for phase timing
-    NttTables tb = {th->roots_f, th->roots_i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    // ev (optional): events recorded after passes 0..3 (ev[0..3]) for phase timing
+    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
     uint64_t N = 1ull << log_n;
+    if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
     uint32_t log_b = log_ext - log_n;
-    // plan: n2 * B <= LDS_ELEMS, n1 = N / n2
-    uint32_t log_lds = NTT_LOG_LDS;
-    if (log_b > log_lds) return -2;
-    uint32_t log_n2 = log_n;
-    if (log_n2 + log_b > log_lds) log_n2 = log_lds > log_b ? log_lds - log_b : 0;
-    uint32_t log_n1 = log_n - log_n2;
-    if (log_n1 > log_lds) return -2;
-    if (log_ext > 24) return -3;
+    if (log_b > NTT_LOG_TILE) return -2;
+    uint64_t n_inv = gl_inv(N % GL_P);
 
-    // 1. row-major trace -> column-major scratch
+    // 0. row-major trace -> column-major scratch
     dim3 tg((unsigned)((N + TR_T - 1) / TR_T), (unsigned)((ncols + TR_T - 1) / TR_T));
     hipLaunchKernelGGL(k_transpose_rm2cm, tg, dim3(256), 0, st, trace, scratch, N, ncols, ncols, N);
     if (ev) hipEventRecord(ev[0], st);
-    // 2. strided inverse DIF
-    if (log_n1 > 0) {
-        uint32_t log_t = log_lds - log_n1;
-        if (log_t > log_n2) log_t = log_n2;
-        dim3 g((unsigned)(1ull << (log_n2 - log_t)), (unsigned)ncols);
-        size_t sh = (size_t)8 << (log_n1 + log_t);
-        hipLaunchKernelGGL(k_ntt_strided<true>, g, dim3(NTT_THREADS), sh, st, scratch, N, log_n1, log_n2, log_t, tb);
+
+    if (log_n < 4) {  // columns shorter than 16 rows
+        uint64_t ne = 1ull << log_ext;
+        hipLaunchKernelGGL(k_lde_tiny, dim3((unsigned)((ne + 255) / 256), (unsigned)ncols), dim3(256), 0, st, scratch,
+                           N, lde, ne, log_n, log_ext, n_inv, gl_inv(kW[log_n]), kW[log_ext]);
+        for (int i = 1; i < 4; i++)
+            if (ev) hipEventRecord(ev[i], st);
+        return (int)hipGetLastError();
+    }
+    // plan: n2 * B <= 2^14 values per block, n1 = N / n2 <= 2^10
+    uint32_t L2 = log_n;
+    if (L2 + log_b > NTT_LOG_TILE) L2 = NTT_LOG_TILE - log_b;
+    uint32_t L1 = log_n - L2;
+    if (L1 > 10 || L2 < 4) return -2;  // n1 <= 2^10 and >= 16 values per block line
+
+    // 1. strided inverse DIF over n1
+    if (L1 > 0) {
+        uint32_t lt = NTT_LOG_TILE - L1;
+        if (lt > L2) lt = L2;
+        uint32_t log_tile = L1 + lt;
+        dim3 g((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
+        hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
+                           L2, lt, tb);
     }
     if (ev) hipEventRecord(ev[1], st);
-    // 3. fused block: DIF, coset scale, extend, DIT, twiddle
-    uint64_t n_inv = gl_inv(N % GL_P);
+    // 2. fused block: DIF, coset scale, spread, DIT, twiddle
     {
-        dim3 g((unsigned)(1u << log_n1), (unsigned)ncols);
-        size_t sh = (size_t)8 << (log_n2 + log_b);
-        hipLaunchKernelGGL(k_lde_block, g, dim3(NTT_THREADS), sh, st, scratch, N, lde, N << log_b, log_n1, log_n2,
-                           log_b, n_inv, tb);
+        uint32_t log_tile = L2 + log_b;
+        dim3 g((unsigned)(1u << L1), (unsigned)ncols);
+        hipLaunchKernelGGL(k_lde_block, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
+                           N << log_b, L1, L2, log_b, n_inv, tb);
     }
     if (ev) hipEventRecord(ev[2], st);
-    // 4. strided forward DIT
-    if (log_n1 > 0) {
-        uint32_t log_n2b = log_n2 + log_b;
-        uint32_t log_t = log_lds - log_n1;
-        if (log_t > log_n2b) log_t = log_n2b;
-        dim3 g((unsigned)(1ull << (log_n2b - log_t)), (unsigned)ncols);
-        size_t sh = (size_t)8 << (log_n1 + log_t);
-        hipLaunchKernelGGL(k_ntt_strided<false>, g, dim3(NTT_THREADS), sh, st, lde, N << log_b, log_n1, log_n2b,
-                           log_t, tb);
+    // 3. strided forward DIT over n1
+    if (L1 > 0) {
+        uint32_t l2b = L2 + log_b;
+        uint32_t lt = NTT_LOG_TILE - L1;
+        if (lt > l2b) lt = l2b;
+        uint32_t log_tile = L1 + lt;
+        dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
+        hipLaunchKernelGGL(k_ntt_strided_fwd, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
+                           N << log_b, L1, l2b, lt, tb);
     }
     if (ev) hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
