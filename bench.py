@@ -31,6 +31,8 @@ METRIC = "commit GElem/s (LDE+Poseidon2 Merkle), batched vs plain; p50/p99 laten
 CONFIGS = {
     "plain_large": dict(n_bits=22, blowup_bits=2, cols=128, arity=3, seed=0x5EED0002),
     "plain_small": dict(n_bits=16, blowup_bits=1, cols=32, arity=3, seed=0x5EED0001),
+    # 64 instances: log2 rows U{12..21}, columns U{8..256}, blowup 2 or 4 (seeded)
+    "batched": dict(n_instances=64, arity=3, seed=0x5EED0003),
 }
 
 # MI355X published peaks
@@ -91,6 +93,173 @@ def cpu_baseline(cfg, target_s=15.0, max_cols=None):
             **host_info()}
 
 
+def cpu_baseline_batched(shapes, arity, target_s=15.0):
+    """Reference CPU path (oracle/_ref, all host cores) over the batch's own
+    instances, taken in generator order while the estimated time stays within
+    about `target_s` seconds (rate estimated from the instances already run)."""
+    import datagen
+    from oracle.oracle import Oracle, RefLib
+    o = Oracle()
+    try:
+        lib, kind = RefLib(), "reference"
+        threads = lib.num_threads()
+    except (OSError, FileNotFoundError):
+        lib, kind = o, "port"
+        threads = o.num_threads()
+    elems, spent, used = 0, 0.0, []
+    for j in shapes:
+        n, ne, c = 1 << j["n_bits"], 1 << j["n_bits_ext"], j["n_cols"]
+        if spent > 0 and spent + ne * c / (elems / spent) > target_s:
+            continue
+        trace = datagen.uniform(n * c, j["seed"]).reshape(n, c)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            lib.merkletree(lib.lde(trace, ne), arity, o.tree_num_elements(ne, arity))
+        else:
+            o.merkletree(o.lde(trace, ne), arity)
+        spent += time.perf_counter() - t0
+        elems += ne * c
+        used.append("2^%d x %d (x%d)" % (j["n_bits"], c, ne // n))
+    return {"value": elems / spent / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+            "sample": "%d of the batch's instances [%s], LDE + Poseidon2 W%d arity %d Merkle, %.2f s"
+                      % (len(used), ", ".join(used), 4 * arity, arity, spent),
+            **host_info()}
+
+
+def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
+    """VALU roofline of the Poseidon2 leaf kernel: algorithmic Goldilocks
+    multiplications (736 per W12 permutation) against the rate at which the VALU
+    can issue the four 32x32->64 partial products each one needs (v_mad_u64_u32
+    at the single-issue lane rate), i.e. a products-only ceiling."""
+    leaf_s = leaf_ms / 1e3
+    muls_per_s = perms * MULS_PER_PERM[width] / leaf_s
+    mul_peak = VALU_PEAK_TOPS / 4.0  # T mul/s
+    return {
+        "bound": "valu", "kernel": "k_p2_leaves<%d>" % width,
+        "achieved": round(muls_per_s / 1e12, 4), "peak": round(mul_peak, 4), "unit": "Tmul/s",
+        "frac": round(muls_per_s / 1e12 / mul_peak, 4),
+        "model": "64x64 field products: %d per W%d permutation, 4 v_mad_u64_u32 each at %.1f T lane-op/s "
+                 "(published 157.3 TFLOP/s FP32 vector / 2 FMA / 2 packed)" % (MULS_PER_PERM[width], width,
+                                                                             VALU_PEAK_TOPS),
+        "perm_per_s": perms / leaf_s,
+        "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "avg_ms": leaf_ms / launches,
+    }
+
+
+def run_batched(args, cfg):
+    """Mixed-size batch: all instances resident in HBM, committed (a) one by one
+    through the plain per-instance path (commit + root back to the host, the way
+    proofman calls commit_witness per instance) and (b) as one hipGraph launch of
+    pil2hip_commit_batch. Both produce every instance's root; they must agree."""
+    import ctypes
+    import torch
+    import datagen
+    import pil2_commit as pc
+
+    arity = cfg["arity"]
+    width = 4 * arity
+    shapes = datagen.batched_workload(seed=cfg["seed"], n_instances=cfg["n_instances"])
+    pc.set_hash_family(2)
+    dev = torch.device("cuda:0")
+    need = 0
+    for j in shapes:
+        n, ne, c = 1 << j["n_bits"], 1 << j["n_bits_ext"], j["n_cols"]
+        need += 8 * (2 * n * c + ne * c + pc.get_tree_num_elements(ne, arity))
+    free, total = torch.cuda.mem_get_info(dev)
+    if need > free - (4 << 30):
+        raise SystemExit("batched workload needs %.1f GB of HBM, %.1f GB free" % (need / 1e9, free / 1e9))
+    jobs = []
+    for j in shapes:
+        tr = pc.empty((1 << j["n_bits"]) * j["n_cols"], dev)
+        pc.fill_uniform(tr, j["seed"])
+        jobs.append(dict(j, trace=tr))
+    batch = pc.CommitBatch(jobs, arity=arity)
+    torch.cuda.synchronize()
+    elems = sum((1 << j["n_bits_ext"]) * j["n_cols"] for j in shapes)
+
+    # (a) plain per-instance path, with phase timing per commit
+    def plain_pass(timed):
+        roots = []
+        for jb in batch.jobs:
+            pc.commit(jb["trace"], arity, jb["n_bits"], jb["n_bits_ext"], jb["n_cols"], lde=jb["lde"],
+                      nodes=jb["nodes"], scratch=jb["scratch"])
+            roots.append(pc.MerkleTreeGL(arity, 0, True, 1 << jb["n_bits_ext"], jb["n_cols"],
+                                         nodes=jb["nodes"]).get_root())
+        return roots
+
+    plain_pass(False)
+    torch.cuda.synchronize()
+    pc.lib.pil2hip_timing_read(None, True)
+    pc.lib.pil2hip_timing_enable(True)
+    t0 = time.perf_counter()
+    plain_roots = plain_pass(True)
+    torch.cuda.synchronize()
+    dt_plain = time.perf_counter() - t0
+    pc.lib.pil2hip_timing_enable(False)
+    ph = (ctypes.c_double * 6)()
+    ncalls = pc.lib.pil2hip_timing_read(ph, True)
+    phase_ms = [ph[i] for i in range(6)]  # sums over the pass
+    plain_value = elems / dt_plain / 1e9
+
+    # (b) batched: first call autotunes the branch count and instantiates the graph
+    t_tune = time.perf_counter()
+    streams = batch.launch(n_streams=args.batch_streams)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t_tune
+    for _ in range(args.warmup):
+        batch.launch(n_streams=streams)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch(n_streams=streams)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = elems / (dt / args.steps) / 1e9
+    roots = batch.roots_host()
+    match = all(list(map(int, roots[k])) == list(map(int, plain_roots[k])) for k in range(len(shapes)))
+
+    rate = width - 4
+    perms = sum((1 << j["n_bits_ext"]) * ((j["n_cols"] + rate - 1) // rate) for j in shapes)
+    leaf_bytes = sum(8 * (1 << j["n_bits_ext"]) * j["n_cols"] + 32 * (1 << j["n_bits_ext"]) for j in shapes)
+    roofline = _leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
+                              _profile_traffic("k_p2_leaves<%d>" % width, "batched"), len(shapes))
+    roofline["measured_on"] = "per-instance commits of the same batch (HIP events around each leaf launch)"
+    lde_bytes = sum(8 * j["n_cols"] * ((1 << j["n_bits"]) + (1 << j["n_bits_ext"])) for j in shapes)
+    lde_ms = sum(phase_ms[0:4])
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks field, exact integer)",
+        "data": "synthetic: seeded uniform Goldilocks elements generated in HBM",
+        "config": {"workload": "batched: %d instances, log2 rows U{12..21}, cols U{8..256}, blowup 2/4 "
+                               "(seed 0x%X), Poseidon2 W%d arity %d Merkle commit each, one hipGraph launch"
+                               % (len(shapes), cfg["seed"], width, arity),
+                   "n_instances": len(shapes), "lde_elements": elems, "arity": arity, "seed": cfg["seed"],
+                   "hbm_resident_GB": round(need / 1e9, 2), "path": "batched (pil2hip_commit_batch)"},
+        "batched": {"graph_branches": streams, "autotune_s": round(t_tune, 3), "roots_match_plain": match},
+        "plain": {"value": round(plain_value, 4), "unit": "GElem/s", "ms_per_pass": round(dt_plain * 1e3, 3),
+                  "path": "per-instance pil2hip_commit + root to host, one instance at a time",
+                  "phases_ms_sum": dict(zip(["transpose", "intt_strided", "lde_block", "ntt_strided", "leaves",
+                                             "tree_levels"], [round(x, 3) for x in phase_ms])),
+                  "timed_commits": ncalls},
+        "batched_over_plain": round(value / plain_value, 4),
+        "roofline": roofline,
+        "roofline_lde": {"bound": "hbm", "achieved": lde_bytes / (lde_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": lde_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": None, "avg_ms": lde_ms / len(shapes)},
+        "device": torch.cuda.get_device_name(0), "host": platform.node(),
+    }
+    if not match:
+        out["error"] = "batched roots differ from the per-instance roots"
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batched(shapes, arity, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    if not match:
+        sys.exit(1)
+
+
 def _profile_traffic(kernel, config):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
@@ -126,6 +295,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate duration of the bounded CPU-baseline sample")
+    ap.add_argument("--batch-streams", type=int, default=0,
+                    help="graph branches for the batched config (0 = autotune)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -140,6 +311,9 @@ def main():
     import pil2_commit as pc
 
     cfg = CONFIGS[args.config]
+    if args.config == "batched":
+        run_batched(args, cfg)
+        return
     nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
     n, ne, width = 1 << nb, 1 << nbe, 4 * arity
     pc.set_hash_family(2)
@@ -176,27 +350,9 @@ def main():
     # dominant kernel: Poseidon2 leaf hashing (k_p2_leaves<W>)
     rate = width - 4
     perms = ne * ((cols + rate - 1) // rate)
-    leaf_s = phase_ms[4] / 1e3
     leaf_bytes = 8 * ne * cols + 32 * ne  # reads every LDE word once, writes one digest per row
-    muls_per_s = perms * MULS_PER_PERM[width] / leaf_s
-    # VALU roofline of the leaf kernel: algorithmic Goldilocks multiplications
-    # (736 per W12 permutation) against the rate at which the VALU can issue the
-    # four 32x32->64 partial products each one needs (v_mad_u64_u32 at the
-    # single-issue lane rate), i.e. a products-only ceiling.
-    mul_peak = VALU_PEAK_TOPS / 4.0  # T mul/s
-    traffic = _profile_traffic("k_p2_leaves<%d>" % width, args.config)
-    roofline = {
-        "bound": "valu", "kernel": "k_p2_leaves<%d>" % width,
-        "achieved": round(muls_per_s / 1e12, 4), "peak": round(mul_peak, 4), "unit": "Tmul/s",
-        "frac": round(muls_per_s / 1e12 / mul_peak, 4),
-        "model": "64x64 field products: %d per W%d permutation, 4 v_mad_u64_u32 each at %.1f T lane-op/s "
-                 "(published 157.3 TFLOP/s FP32 vector / 2 FMA / 2 packed)" % (MULS_PER_PERM[width], width,
-                                                                             VALU_PEAK_TOPS),
-        "perm_per_s": perms / leaf_s,
-        "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
-        "traffic": traffic,
-        "avg_ms": phase_ms[4],
-    }
+    roofline = _leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
+                              _profile_traffic("k_p2_leaves<%d>" % width, args.config), 1)
     # LDE passes: algorithmic bytes 8*C*(N + NExt) over the four LDE kernels
     lde_ms = sum(phase_ms[0:4])
     lde_bytes = 8 * cols * (n + ne)

@@ -87,6 +87,33 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
 
+/* One commit of a batch: the arguments of pil2hip_commit. Buffers are device
+ * memory owned by the caller and must not overlap between jobs. */
+typedef struct Pil2CommitJob {
+    const uint64_t *d_trace; /* row-major 2^n_bits x n_cols */
+    uint64_t *d_lde;         /* column-major 2^n_bits_ext x n_cols */
+    uint64_t *d_nodes;       /* pil2hip_tree_num_elements(2^n_bits_ext, arity) words */
+    uint64_t *d_scratch;     /* 2^n_bits x n_cols words */
+    uint64_t n_bits, n_bits_ext, n_cols, arity;
+} Pil2CommitJob;
+
+/* Batched commit of many AIR instances of uneven shapes: the per-instance
+ * commits that proofman issues one by one (commit_witness, starks_api.hpp:98,
+ * once per instance from proofman.rs) run as ONE hipGraph launch on `stream`.
+ * Jobs are spread over n_streams concurrent branches of the graph (longest
+ * first, each to the least-loaded branch); each job's root is copied to
+ * d_roots[4*i .. 4*i+4) (d_roots may be NULL). The instantiated graph is cached
+ * per job list (pointers and shapes) and replayed on later calls.
+ * n_streams = 0 autotunes: on the first call for a job list, 1, 2, 4, 8 and 16
+ * branches are each timed over one replay and the fastest is kept (these runs
+ * recompute the same outputs). Returns the branch count used (> 0), 0 for an
+ * empty batch, or a negative error code. Asynchronous like pil2hip_commit. */
+int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_t n_streams, uint64_t *d_roots,
+                             void *stream);
+
+/* Destroys every cached batch graph. */
+void pil2hip_batch_release(void);
+
 /* MerkleTreeGL::getRoot (merkleTreeGL.cpp:103-107): synchronous copy of the root
  * (4 words) into host memory. */
 int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, void *stream);

@@ -187,7 +187,19 @@ __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
     for (int k = 0; k < W; k++) st[i * W + k] = s[k];
 }
 
+// Copies one 4-word root (used to gather the roots of a batched commit).
+__global__ void k_copy_root(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst)
+{
+    if (threadIdx.x < 4) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 // ------------------------------------------------------------ launchers ----
+extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_root, dim3(1), dim3(64), 0, st, src, dst);
+    return (int)hipGetLastError();
+}
+
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st)
 {

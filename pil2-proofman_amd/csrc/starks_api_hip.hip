@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -113,9 +114,11 @@ static hipEvent_t g_ev[kPhases + 1];
 static double g_phase_ms[kPhases];
 static uint64_t g_timed_calls = 0;
 
+static thread_local bool t_capturing = false;  // no timing events inside graph capture
+
 static hipEvent_t *timing_events()
 {
-    if (!g_timing) return nullptr;
+    if (!g_timing || t_capturing) return nullptr;
     if (!g_timing_ev_ready) {
         for (int i = 0; i <= kPhases; i++)
             if (hipEventCreate(&g_ev[i]) != hipSuccess) return nullptr;
@@ -242,6 +245,7 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
     int64_t rc = check_family();
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t *ev = timing_events();
     if (ev) hipEventRecord(g_ev[0], st);
@@ -263,6 +267,179 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
         g_timed_calls++;
     }
     return 0;
+}
+
+// ------------------------------------------------------------ batched commit --
+// Many independent commits (uneven shapes) in one hipGraph: every job's
+// LDE + Merkle chain is enqueued on one of S internal streams (longest jobs
+// first, each to the least-loaded stream), a one-block kernel copies each root
+// into d_roots, and the whole fork/join capture is instantiated once per job
+// list and replayed. S = 0 lets the autotuner time S in {1, 2, 4, 8, 16} on the
+// first call and keep the fastest for this job list.
+struct BatchGraph {
+    std::vector<uint64_t> key;
+    uint32_t streams;
+    hipGraphExec_t exec;
+};
+static std::vector<BatchGraph> g_batch_graphs;
+static std::vector<hipStream_t> g_batch_streams;
+static std::vector<hipEvent_t> g_batch_events;
+static hipStream_t g_capture_stream = nullptr;
+static const uint32_t kMaxBatchStreams = 16;
+static std::mutex g_batch_mu;  // batch graphs, streams and events are shared state
+
+static double job_work(const Pil2CommitJob &j)
+{
+    // LDE element count weighted by the sponge permutations per row
+    double rate = (double)(4 * j.arity - 4);
+    return (double)(1ull << j.n_bits_ext) * (double)j.n_cols * (1.0 + 8.0 / rate);
+}
+
+static int64_t batch_resources()
+{
+    if (g_capture_stream) return 0;
+    HIP_TRY(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking), "capture stream");
+    g_batch_streams.resize(kMaxBatchStreams);
+    g_batch_events.resize(kMaxBatchStreams + 1);
+    for (uint32_t i = 0; i < kMaxBatchStreams; i++)
+        HIP_TRY(hipStreamCreateWithFlags(&g_batch_streams[i], hipStreamNonBlocking), "batch stream");
+    for (auto &e : g_batch_events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch event");
+    return 0;
+}
+
+static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, uint64_t *d_roots,
+                             hipGraphExec_t *out)
+{
+    // LPT assignment: jobs by decreasing work, each to the least-loaded stream
+    std::vector<uint64_t> order(n);
+    for (uint64_t i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return job_work(jobs[a]) > job_work(jobs[b]); });
+    std::vector<double> load(S, 0.0);
+    std::vector<uint32_t> assign(n);
+    for (uint64_t k = 0; k < n; k++) {
+        uint32_t best = 0;
+        for (uint32_t s = 1; s < S; s++)
+            if (load[s] < load[best]) best = s;
+        assign[order[k]] = best;
+        load[best] += job_work(jobs[order[k]]);
+    }
+    hipStream_t cap = g_capture_stream;
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+    t_capturing = true;
+    int64_t rc = 0;
+    hipEventRecord(g_batch_events[kMaxBatchStreams], cap);
+    for (uint32_t s = 0; s < S; s++) hipStreamWaitEvent(g_batch_streams[s], g_batch_events[kMaxBatchStreams], 0);
+    for (uint64_t k = 0; k < n && !rc; k++) {
+        const Pil2CommitJob &j = jobs[order[k]];
+        hipStream_t st = g_batch_streams[assign[order[k]]];
+        rc = pil2hip_commit(j.d_lde, j.d_nodes, j.d_trace, j.d_scratch, j.arity, j.n_bits, j.n_bits_ext, j.n_cols, st);
+        if (!rc && d_roots) {
+            uint64_t nt = pil2hip_tree_num_elements(1ull << j.n_bits_ext, j.arity);
+            int e = p2_launch_copy_root(j.d_nodes + nt - 4, d_roots + 4 * order[k], st);
+            if (e) rc = fail(-100 - e, "root copy launch failed");
+        }
+    }
+    for (uint32_t s = 0; s < S; s++) {
+        hipEventRecord(g_batch_events[s], g_batch_streams[s]);
+        hipStreamWaitEvent(cap, g_batch_events[s], 0);
+    }
+    t_capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t ee = hipStreamEndCapture(cap, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("end capture: ") + hipGetErrorString(ee));
+    ee = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("graph instantiate: ") + hipGetErrorString(ee));
+    return 0;
+}
+
+int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_t n_streams, uint64_t *d_roots,
+                             void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if (n_jobs == 0) return 0;
+    if (!jobs) return fail(-13, "null jobs");
+    if (n_streams > kMaxBatchStreams) return fail(-27, "n_streams > 16");
+    for (uint64_t i = 0; i < n_jobs; i++) {
+        const Pil2CommitJob &j = jobs[i];
+        if ((rc = check_arity(j.arity))) return rc;
+        if (j.n_bits_ext < j.n_bits || j.n_bits_ext > NTT_MAX_LOG) return fail(-10, "bad nBits/nBitsExt in a job");
+        if (!j.d_trace || !j.d_lde || !j.d_nodes || !j.d_scratch) return fail(-13, "null buffer in a job");
+    }
+    NttTablesHost *tb;
+    if ((rc = ensure_tables(&tb))) return rc;  // allocations happen before any capture
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    if ((rc = batch_resources())) return rc;
+    std::vector<uint64_t> key;
+    key.push_back((uint64_t)(uintptr_t)d_roots);
+    for (uint64_t i = 0; i < n_jobs; i++) {
+        const Pil2CommitJob &j = jobs[i];
+        key.insert(key.end(), {(uint64_t)(uintptr_t)j.d_trace, (uint64_t)(uintptr_t)j.d_lde,
+                               (uint64_t)(uintptr_t)j.d_nodes, (uint64_t)(uintptr_t)j.d_scratch, j.n_bits,
+                               j.n_bits_ext, j.n_cols, j.arity});
+    }
+    hipStream_t st = (hipStream_t)stream;
+    for (auto &g : g_batch_graphs)
+        if (g.key == key && (n_streams == 0 || g.streams == n_streams)) {
+            HIP_TRY(hipGraphLaunch(g.exec, st), "graph launch");
+            return g.streams;
+        }
+    std::vector<uint32_t> cands;
+    if (n_streams) cands.push_back(n_streams);
+    else
+        for (uint32_t s = 1; s <= kMaxBatchStreams && s <= std::max<uint64_t>(n_jobs, 1); s *= 2) cands.push_back(s);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    HIP_TRY(hipEventCreate(&e1), "event");
+    float best_ms = 1e30f;
+    hipGraphExec_t best = nullptr;
+    uint32_t best_s = 0;
+    for (uint32_t S : cands) {
+        hipGraphExec_t ex = nullptr;
+        if ((rc = batch_capture(jobs, n_jobs, S, d_roots, &ex))) break;
+        float ms = 0;
+        if (cands.size() > 1) {  // autotune: one warm launch, one timed launch
+            hipGraphLaunch(ex, st);
+            hipEventRecord(e0, st);
+            hipGraphLaunch(ex, st);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (ms < best_ms) {
+            if (best) hipGraphExecDestroy(best);
+            best = ex;
+            best_ms = ms;
+            best_s = S;
+        } else {
+            hipGraphExecDestroy(ex);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) {
+        if (best) hipGraphExecDestroy(best);
+        return rc;
+    }
+    if (g_batch_graphs.size() >= 8) {  // small cache: drop the oldest graph
+        hipGraphExecDestroy(g_batch_graphs.front().exec);
+        g_batch_graphs.erase(g_batch_graphs.begin());
+    }
+    g_batch_graphs.push_back({key, best_s, best});
+    HIP_TRY(hipGraphLaunch(best, st), "graph launch");
+    return best_s;
+}
+
+void pil2hip_batch_release(void)
+{
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    for (auto &g : g_batch_graphs) hipGraphExecDestroy(g.exec);
+    g_batch_graphs.clear();
 }
 
 void pil2hip_timing_enable(bool on) { g_timing = on; }

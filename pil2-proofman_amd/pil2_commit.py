@@ -48,6 +48,8 @@ _SIGS = {
     "pil2hip_lde": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _vp]),
     "pil2hip_merkelize": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
+    "pil2hip_batch_release": (None, []),
     "pil2hip_get_root": (_i64, [_vp, _vp, _u64, _u64, _vp]),
     "pil2hip_fri_fold": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_fri_transpose": (_i64, [_vp, _vp, _u64, _u64, _vp]),
@@ -308,3 +310,55 @@ def commit(trace_dev, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, s
     _check(_lib.pil2hip_commit(_ptr(lde), _ptr(nodes), _ptr(trace_dev), _ptr(scratch), arity, n_bits,
                                n_bits_ext, n_cols, _stream(stream)))
     return lde, nodes
+
+
+class Pil2CommitJob(ctypes.Structure):
+    """Mirror of the C struct Pil2CommitJob (include/pil2_commit_hip.h)."""
+    _fields_ = [("d_trace", _vp), ("d_lde", _vp), ("d_nodes", _vp), ("d_scratch", _vp),
+                ("n_bits", _u64), ("n_bits_ext", _u64), ("n_cols", _u64), ("arity", _u64)]
+
+
+class CommitBatch:
+    """Many independent commits launched as one hipGraph (pil2hip_commit_batch).
+
+    `jobs` is a list of dicts with keys trace (device tensor, row-major
+    2^n_bits x n_cols), n_bits, n_bits_ext, n_cols and optionally lde, nodes,
+    scratch (allocated here when absent). The tensors are kept alive by this
+    object because the library caches the captured graph by their addresses."""
+
+    def __init__(self, jobs, arity=3, device=None):
+        self.jobs = []
+        arr = (Pil2CommitJob * max(len(jobs), 1))()
+        for i, j in enumerate(jobs):
+            dev = j["trace"].device if device is None else device
+            n, ne, nc = 1 << j["n_bits"], 1 << j["n_bits_ext"], j["n_cols"]
+            lde = j.get("lde")
+            lde = empty(ne * nc, dev) if lde is None else lde
+            nodes = j.get("nodes")
+            nodes = empty(get_tree_num_elements(ne, arity), dev) if nodes is None else nodes
+            scratch = j.get("scratch")
+            scratch = empty(max(n * nc, 1), dev) if scratch is None else scratch
+            rec = dict(j, lde=lde, nodes=nodes, scratch=scratch, arity=arity)
+            self.jobs.append(rec)
+            arr[i] = Pil2CommitJob(j["trace"].data_ptr(), lde.data_ptr(), nodes.data_ptr(), scratch.data_ptr(),
+                                   j["n_bits"], j["n_bits_ext"], nc, arity)
+        self._arr = arr
+        dev = jobs[0]["trace"].device if jobs else "cuda"
+        self.roots = empty(4 * max(len(jobs), 1), dev)
+        self.streams = None
+
+    def launch(self, n_streams=0, stream=None):
+        """Enqueue the batch; returns the number of graph branches used."""
+        rc = _lib.pil2hip_commit_batch(ctypes.cast(self._arr, _vp), len(self.jobs), n_streams,
+                                       _ptr(self.roots), _stream(stream))
+        if rc < 0:
+            _check(rc)
+        self.streams = int(rc)
+        return self.streams
+
+    def roots_host(self):
+        return to_host(self.roots)[:4 * len(self.jobs)].reshape(-1, 4)
+
+
+def batch_release():
+    _lib.pil2hip_batch_release()
