@@ -33,6 +33,8 @@ CONFIGS = {
     "plain_small": dict(n_bits=16, blowup_bits=1, cols=32, arity=3, seed=0x5EED0001),
     # 64 instances: log2 rows U{12..21}, columns U{8..256}, blowup 2 or 4 (seeded)
     "batched": dict(n_instances=64, arity=3, seed=0x5EED0003),
+    # Poisson stream of 2000 jobs (log2 rows U{12..22}), HBM budget 200 GB, 32 openings per job
+    "serving": dict(n_jobs=2000, arity=3, seed=0x5EED0005, budget=200e9, queries=32, llv=2, load=1.2),
 }
 
 # MI355X published peaks
@@ -260,6 +262,105 @@ def run_batched(args, cfg):
         sys.exit(1)
 
 
+def run_serving(args, cfg):
+    """Serving: jobs arrive as a Poisson stream and are admitted by the commit
+    server (pil2hip_serve_*) under an HBM budget while earlier jobs still run;
+    each job commits its trace (synthetic values generated in HBM from its seed)
+    and returns its root and 32 group proofs (getGroupProof, llv 2). The arrival
+    rate is 1.2x the throughput the same server sustains on a closed burst of
+    jobs. Reports aggregate GElem/s and p50/p99 of completion - arrival."""
+    import numpy as np
+    import torch
+    import datagen
+    import pil2_commit as pc
+
+    arity, llv, nq = cfg["arity"], cfg["llv"], cfg["queries"]
+    n_jobs = args.jobs or cfg["n_jobs"]
+    shapes = datagen.batched_workload(seed=cfg["seed"], n_instances=n_jobs, min_log=12, max_log=22)
+    qwords = datagen.raw_words(n_jobs * nq, cfg["seed"] ^ 0xA5A5A5A5).reshape(n_jobs, nq)
+    pc.set_hash_family(2)
+    free, _ = torch.cuda.mem_get_info(torch.device("cuda:0"))
+    budget = int(min(cfg["budget"], free - (8 << 30)))
+    eng = pc.ServeEngine(budget, n_streams=args.serve_streams, max_inflight=args.serve_inflight, arity=arity,
+                         llv=llv)
+    elems = [(1 << j["n_bits_ext"]) * j["n_cols"] for j in shapes]
+
+    def submit(k):
+        j = shapes[k]
+        q = qwords[k] % np.uint64(1 << j["n_bits_ext"])
+        return eng.submit(j["n_bits"], j["n_bits_ext"], j["n_cols"], queries=q, seed=j["seed"], tag=k)
+
+    # capacity: a closed burst of the first jobs, all submitted at once
+    n_cal = min(48, n_jobs)
+    t0 = time.perf_counter()
+    for k in range(n_cal):
+        submit(k)
+    got = 0
+    while got < n_cal:
+        for rec in eng.poll(max_n=256, timeout_ms=200):
+            assert rec["status"] == 0, rec["status"]
+            eng.forget(rec["job_id"])
+            got += 1
+    cal_s = time.perf_counter() - t0
+    capacity = sum(elems[:n_cal]) / cal_s  # elements per second
+    lam = cfg["load"] * capacity / (sum(elems) / n_jobs)  # jobs per second
+    rng = np.random.default_rng(cfg["seed"])
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
+
+    lat, qdelay, done, bad = [], [], 0, 0
+    t_first = t_last = None
+    start = time.perf_counter()
+    base_us = eng.now_us()
+    k = 0
+    last_progress = start
+    while done < n_jobs:
+        now = time.perf_counter() - start
+        while k < n_jobs and arrivals[k] <= now:
+            submit(k)
+            k += 1
+        wait_ms = 0 if k >= n_jobs else max(0, int((arrivals[k] - (time.perf_counter() - start)) * 1e3))
+        for rec in eng.poll(max_n=256, timeout_ms=min(wait_ms, 50) if k < n_jobs else 100):
+            if rec["status"] != 0:
+                bad += 1
+            lat.append((rec["t_done_us"] - rec["t_submit_us"]) / 1e3)
+            qdelay.append((rec["t_admit_us"] - rec["t_submit_us"]) / 1e3)
+            t_first = rec["t_submit_us"] if t_first is None else min(t_first, rec["t_submit_us"])
+            t_last = rec["t_done_us"] if t_last is None else max(t_last, rec["t_done_us"])
+            eng.forget(rec["job_id"])
+            done += 1
+        if time.perf_counter() - last_progress > 20:
+            last_progress = time.perf_counter()
+            print("serving: %d/%d submitted, %d done" % (k, n_jobs, done), file=sys.stderr, flush=True)
+    span_s = (t_last - min(t_first, base_us)) / 1e6
+    st = eng.stats()
+    eng.close()
+    lat = np.array(lat)
+    value = sum(elems) / span_s / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(span_s * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (Goldilocks field, exact integer)",
+        "data": "synthetic: each job's uniform trace generated in HBM from its seed at admission",
+        "config": {"workload": "serving: %d Poisson jobs (log2 rows U{12..22}, cols U{8..256}, blowup 2/4, "
+                               "seed 0x%X), HBM budget %.0f GB, %d group proofs per job (llv %d), "
+                               "offered load %.1fx capacity" % (n_jobs, cfg["seed"], budget / 1e9, nq, llv,
+                                                                cfg["load"]),
+                   "n_jobs": n_jobs, "arity": arity, "streams": args.serve_streams,
+                   "max_inflight": args.serve_inflight, "path": "served (pil2hip_serve_*)"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
+        "queue_delay_ms": {"p50": round(float(np.percentile(qdelay, 50)), 3),
+                           "p99": round(float(np.percentile(qdelay, 99)), 3)},
+        "capacity_GElem_s": round(capacity / 1e9, 4), "arrival_rate_jobs_s": round(lam, 3),
+        "lde_elements": int(sum(elems)), "failed_jobs": bad,
+        "server": {k2: st[k2] for k2 in ("admitted", "completed", "failed", "peak_bytes", "peak_running")},
+        "device": torch.cuda.get_device_name(0), "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+    if bad:
+        sys.exit(1)
+
+
 def _profile_traffic(kernel, config):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
@@ -297,6 +398,9 @@ def main():
                     help="approximate duration of the bounded CPU-baseline sample")
     ap.add_argument("--batch-streams", type=int, default=0,
                     help="graph branches for the batched config (0 = autotune)")
+    ap.add_argument("--jobs", type=int, default=0, help="serving: number of jobs (default 2000)")
+    ap.add_argument("--serve-streams", type=int, default=4, help="serving: device streams")
+    ap.add_argument("--serve-inflight", type=int, default=32, help="serving: admitted jobs in flight")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -313,6 +417,9 @@ def main():
     cfg = CONFIGS[args.config]
     if args.config == "batched":
         run_batched(args, cfg)
+        return
+    if args.config == "serving":
+        run_serving(args, cfg)
         return
     nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
     n, ne, width = 1 << nb, 1 << nbe, 4 * arity

@@ -114,6 +114,91 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
 /* Destroys every cached batch graph. */
 void pil2hip_batch_release(void);
 
+/* MerkleTreeGL::getGroupProof (merkleTreeGL.cpp:145-175) for nQueries row
+ * indices d_idx (device memory) at once: proof q occupies
+ * d_out[q*pw .. (q+1)*pw), pw = pil2hip_proof_words(nRows, nCols, arity, llv):
+ * the nCols values of row idx (element (r, c) at d_src[c*srcColStride + r], the
+ * column-major LDE layout), then per tree level above the last-level cut
+ * (genMerkleProof, merkleTreeGL.cpp:157-175) the arity-1 sibling digests in
+ * ascending order. An index >= nRows yields an all-zero proof. */
+int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t srcColStride, uint64_t nCols,
+                             const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+                             const uint64_t *d_idx, uint64_t nQueries, void *stream);
+
+/* ---------------------------------------------------------------- serving --
+ * A commit server: jobs arrive one by one (pil2hip_serve_submit), are admitted
+ * first-come first-served as in-flight slots and HBM budget free up, run
+ * concurrently on several device streams (LDE, Merkle tree, the job's group
+ * proofs), and report completions (pil2hip_serve_poll) with their timestamps.
+ * A job's LDE and tree can stay resident for later openings
+ * (pil2hip_serve_open) until pil2hip_serve_release. The reference's analogue is
+ * proofman's per-instance commit_witness calls and its stream/slot API
+ * (configure_stream_commit_slots / commit_witness_streaming,
+ * starks_api.hpp:214-216); this engine admits by HBM budget instead. */
+typedef struct Pil2ServeConfig {
+    uint64_t hbm_budget_bytes; /* device arena holding every admitted job's buffers */
+    uint32_t n_streams;        /* concurrent device streams, 1..16 */
+    uint32_t max_inflight;     /* admitted, not yet completed jobs, 1..256 */
+    uint64_t arity;            /* Merkle arity, 2..4 (Poseidon2 width 4*arity) */
+    uint64_t llv;              /* last level verification of the group proofs */
+} Pil2ServeConfig;
+
+typedef struct Pil2ServeRequest {
+    uint64_t n_bits, n_bits_ext, n_cols;
+    const uint64_t *h_trace;   /* host row-major trace, read before completion, or NULL */
+    const uint64_t *d_trace;   /* device row-major trace owned by the caller, or NULL */
+    uint64_t seed;             /* both NULL: synthetic trace pil2hip_fill_uniform(seed) */
+    uint64_t n_queries;        /* group proofs computed with the commit */
+    const uint64_t *h_queries; /* row indices < 2^n_bits_ext (copied at submit) */
+    uint64_t *h_root;          /* 4 words, written before the completion is reported */
+    uint64_t *h_proofs;        /* n_queries * pil2hip_proof_words(2^n_bits_ext, n_cols, arity, llv), or NULL */
+    uint64_t tag;              /* returned in the completion */
+    uint32_t keep_resident;    /* 1: LDE and tree stay in HBM until pil2hip_serve_release */
+} Pil2ServeRequest;
+
+typedef struct Pil2ServeCompletion {
+    uint64_t job_id, tag;
+    int64_t status; /* 0, or a negative error code (-31: cancelled) */
+    double t_submit_us, t_admit_us, t_done_us; /* engine clock, pil2hip_serve_now_us */
+    uint32_t stream;
+} Pil2ServeCompletion;
+
+typedef struct Pil2ServeStats {
+    uint64_t submitted, admitted, completed, failed, cancelled, queued, running, resident_jobs;
+    uint64_t bytes_in_use, peak_bytes, budget_bytes;
+    uint32_t peak_running;
+} Pil2ServeStats;
+
+/* Creates a server on the current device (allocates the arena, streams and a
+ * scheduler thread). NULL on error (pil2hip_last_error gives the reason). */
+void *pil2hip_serve_create(const Pil2ServeConfig *cfg);
+/* Queues a job; returns its id (>= 0) or a negative code: -30 when the job
+ * can never fit the HBM budget, -10/-11/-12 for a bad shape, -28 for a query
+ * index out of range. Never blocks on the device. */
+int64_t pil2hip_serve_submit(void *srv, const Pil2ServeRequest *req);
+/* Copies up to max completions into out; waits up to timeout_ms for the first
+ * one. Returns the number copied. */
+int64_t pil2hip_serve_poll(void *srv, Pil2ServeCompletion *out, uint64_t max, int32_t timeout_ms);
+/* Cancels a queued job (completion status -31); -32 if it is already admitted. */
+int64_t pil2hip_serve_cancel(void *srv, uint64_t job_id);
+/* Group proofs on a completed, resident job (synchronous). */
+int64_t pil2hip_serve_open(void *srv, uint64_t job_id, const uint64_t *h_queries, uint64_t n_queries,
+                           uint64_t *h_proofs);
+/* Frees a completed job's resident LDE and tree. */
+int64_t pil2hip_serve_release(void *srv, uint64_t job_id);
+int64_t pil2hip_serve_stats(void *srv, Pil2ServeStats *out);
+double pil2hip_serve_now_us(void *srv);
+/* HBM bytes a job holds while admitted (LDE, tree, trace, scratch, queries, proofs). */
+uint64_t pil2hip_serve_job_bytes(uint64_t n_bits, uint64_t n_bits_ext, uint64_t n_cols, uint64_t arity,
+                                 uint64_t n_queries, uint64_t llv, bool device_trace);
+/* Drains admitted jobs, cancels queued ones, frees everything. */
+void pil2hip_serve_destroy(void *srv);
+
+/* Merkle proof levels (MerkleTreeGL::getMerkleProofLength, merkleTreeGL.cpp:50-55)
+ * and words per group proof (nCols + (arity-1)*4*levels). */
+uint64_t pil2hip_proof_levels(uint64_t nRows, uint64_t arity, uint64_t llv);
+uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uint64_t llv);
+
 /* MerkleTreeGL::getRoot (merkleTreeGL.cpp:103-107): synchronous copy of the root
  * (4 words) into host memory. */
 int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, void *stream);

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpil2_commit_hip.so")
-SOURCES = ["ntt_kernels.hip", "poseidon2_kernels.hip", "fri_kernels.hip", "starks_api_hip.hip"]
+SOURCES = ["ntt_kernels.hip", "poseidon2_kernels.hip", "fri_kernels.hip", "starks_api_hip.hip", "serve_engine.hip"]
 ARCH = os.environ.get("PIL2HIP_ARCH", "gfx950")
 
 FLAGS = [

@@ -10,3 +10,6 @@ extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs
 extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st);
+extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t ld, uint64_t ncols, const uint64_t *tree,
+                                      uint64_t nrows, uint64_t arity, uint64_t cut, const uint64_t *idx,
+                                      uint64_t nq, uint64_t pw, uint64_t *out, hipStream_t st);

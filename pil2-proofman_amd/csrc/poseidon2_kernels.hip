@@ -187,6 +187,43 @@ __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
     for (int k = 0; k < W; k++) st[i * W + k] = s[k];
 }
 
+// ------------------------------------------------------------ openings ----
+// Group proofs (MerkleTreeGL::getGroupProof, merkleTreeGL.cpp:145-175) for many
+// queries: block q writes proof q at out + q*pw: the nCols values of LDE row
+// idx (column-major source, element (r, c) at src[c*ld + r]), then for every
+// level above the last-level-verification cut the arity-1 sibling digests in
+// ascending child order. A query index >= nrows yields an all-zero proof.
+__global__ __launch_bounds__(256) void k_group_proofs(const uint64_t *__restrict__ src, uint64_t ld, uint64_t ncols,
+                                                      const uint64_t *__restrict__ tree, uint64_t nrows,
+                                                      uint64_t arity, uint64_t cut, const uint64_t *__restrict__ idx,
+                                                      uint64_t pw, uint64_t *__restrict__ out)
+{
+    uint64_t q = blockIdx.x;
+    uint64_t r = idx[q];
+    uint64_t *o = out + q * pw;
+    if (r >= nrows) {
+        for (uint64_t k = threadIdx.x; k < pw; k += blockDim.x) o[k] = 0;
+        return;
+    }
+    for (uint64_t c = threadIdx.x; c < ncols; c += blockDim.x) o[c] = gl_canon(src[c * ld + r]);
+    // siblings: level by level; thread t copies word t of each level's (arity-1)*4 words
+    uint64_t per = (arity - 1) * 4;
+    uint64_t t = threadIdx.x;
+    if (t >= per) return;
+    uint64_t n = nrows, off = 0, i = r, pos = ncols;
+    while (n > cut) {
+        uint64_t cur = i % arity, si = i - cur;
+        uint64_t sib = t / 4, w = t % 4;
+        uint64_t child = sib < cur ? sib : sib + 1;  // skip the node on the path
+        o[pos + t] = tree[(off + si + child) * 4 + w];
+        uint64_t nn = (n + arity - 1) / arity;
+        off += nn * arity;
+        i /= arity;
+        n = nn;
+        pos += per;
+    }
+}
+
 // Copies one 4-word root (used to gather the roots of a batched commit).
 __global__ void k_copy_root(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst)
 {
@@ -194,6 +231,16 @@ __global__ void k_copy_root(const uint64_t *__restrict__ src, uint64_t *__restri
 }
 
 // ------------------------------------------------------------ launchers ----
+extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t ld, uint64_t ncols, const uint64_t *tree,
+                                      uint64_t nrows, uint64_t arity, uint64_t cut, const uint64_t *idx,
+                                      uint64_t nq, uint64_t pw, uint64_t *out, hipStream_t st)
+{
+    if (nq == 0) return 0;
+    hipLaunchKernelGGL(k_group_proofs, dim3((unsigned)nq), dim3(256), 0, st, src, ld, ncols, tree, nrows, arity, cut,
+                       idx, pw, out);
+    return (int)hipGetLastError();
+}
+
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st)
 {
     hipLaunchKernelGGL(k_copy_root, dim3(1), dim3(64), 0, st, src, dst);

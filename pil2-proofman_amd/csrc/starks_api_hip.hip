@@ -140,6 +140,64 @@ static int64_t check_arity(uint64_t arity)
     return 0;
 }
 
+static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits,
+                        uint64_t nBitsExt, uint64_t nCols, hipStream_t st, hipEvent_t *ev)
+{
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nCols == 0) return 0;
+    if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
+    if (!d_lde || !d_trace || !d_scratch) return fail(-13, "null buffer");
+    NttTablesHost *tb;
+    int64_t rc = ensure_tables(&tb);
+    if (rc) return rc;
+    int e = ntt_launch_lde(d_lde, d_trace, d_scratch, (uint32_t)nBits, (uint32_t)nBitsExt, nCols, tb, st, ev);
+    if (e < 0) return fail(-14, "LDE plan not supported for this shape");
+    if (e) return fail(-100 - e, std::string("LDE launch: ") + hipGetErrorString((hipError_t)e));
+    return 0;
+}
+
+// LDE + leaves + tree levels on stream st. With ev (the phase events of
+// g_ev[1..]), the phases are timed and the stream is synchronised.
+static int64_t commit_impl(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch,
+                           uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, hipStream_t st,
+                           hipEvent_t *ev)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (!d_nodes) return fail(-13, "null buffer");
+    if (ev) hipEventRecord(g_ev[0], st);
+    if ((rc = lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st, ev))) return rc;
+    uint64_t NExt = 1ull << nBitsExt;
+    uint32_t width = (uint32_t)(4 * arity);
+    int e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
+    if (e) return fail(-100 - e, "leaf hashing launch failed");
+    if (ev) hipEventRecord(g_ev[5], st);
+    e = p2_launch_tree_levels(width, d_nodes, NExt, st);
+    if (e) return fail(-100 - e, "tree level launch failed");
+    if (ev) {
+        hipEventRecord(g_ev[6], st);
+        HIP_TRY(hipEventSynchronize(g_ev[6]), "timing sync");
+        for (int i = 0; i < kPhases; i++) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, g_ev[i], g_ev[i + 1]) == hipSuccess) g_phase_ms[i] += ms;
+        }
+        g_timed_calls++;
+    }
+    return 0;
+}
+
+// Internal interface for the other host sources of the library (no timing).
+int64_t commit_enqueue(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch,
+                       uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, hipStream_t st)
+{
+    return commit_impl(d_lde, d_nodes, d_trace, d_scratch, arity, nBits, nBitsExt, nCols, st, nullptr);
+}
+
+int64_t api_fail(int64_t code, const std::string &msg) { return fail(code, msg); }
+
 // ---------------------------------------------------------- entry points --
 extern "C" {
 
@@ -208,19 +266,7 @@ uint64_t pil2hip_tree_num_elements(uint64_t rows, uint64_t arity)
 int64_t pil2hip_lde(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt,
                     uint64_t nCols, void *stream)
 {
-    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
-    if (nCols == 0) return 0;
-    if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
-    if (!d_lde || !d_trace || !d_scratch) return fail(-13, "null buffer");
-    NttTablesHost *tb;
-    int64_t rc = ensure_tables(&tb);
-    if (rc) return rc;
-    int e = ntt_launch_lde(d_lde, d_trace, d_scratch, (uint32_t)nBits, (uint32_t)nBitsExt, nCols, tb,
-                           (hipStream_t)stream, timing_events());
-    if (e < 0) return fail(-14, "LDE plan not supported for this shape");
-    if (e) return fail(-100 - e, std::string("LDE launch: ") + hipGetErrorString((hipError_t)e));
-    return 0;
+    return lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, (hipStream_t)stream, timing_events());
 }
 
 int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRows, uint64_t nCols, uint64_t rowStride,
@@ -242,31 +288,8 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream)
 {
-    int64_t rc = check_family();
-    if (rc) return rc;
-    if ((rc = check_arity(arity))) return rc;
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
-    hipStream_t st = (hipStream_t)stream;
-    hipEvent_t *ev = timing_events();
-    if (ev) hipEventRecord(g_ev[0], st);
-    if ((rc = pil2hip_lde(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, stream))) return rc;
-    uint64_t NExt = 1ull << nBitsExt;
-    uint32_t width = (uint32_t)(4 * arity);
-    int e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
-    if (e) return fail(-100 - e, "leaf hashing launch failed");
-    if (ev) hipEventRecord(g_ev[5], st);
-    e = p2_launch_tree_levels(width, d_nodes, NExt, st);
-    if (e) return fail(-100 - e, "tree level launch failed");
-    if (ev) {
-        hipEventRecord(g_ev[6], st);
-        HIP_TRY(hipEventSynchronize(g_ev[6]), "timing sync");
-        for (int i = 0; i < kPhases; i++) {
-            float ms = 0;
-            if (hipEventElapsedTime(&ms, g_ev[i], g_ev[i + 1]) == hipSuccess) g_phase_ms[i] += ms;
-        }
-        g_timed_calls++;
-    }
-    return 0;
+    return commit_impl(d_lde, d_nodes, d_trace, d_scratch, arity, nBits, nBitsExt, nCols, (hipStream_t)stream,
+                       timing_events());
 }
 
 // ------------------------------------------------------------ batched commit --
@@ -332,7 +355,8 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, 
     for (uint64_t k = 0; k < n && !rc; k++) {
         const Pil2CommitJob &j = jobs[order[k]];
         hipStream_t st = g_batch_streams[assign[order[k]]];
-        rc = pil2hip_commit(j.d_lde, j.d_nodes, j.d_trace, j.d_scratch, j.arity, j.n_bits, j.n_bits_ext, j.n_cols, st);
+        rc = commit_impl(j.d_lde, j.d_nodes, j.d_trace, j.d_scratch, j.arity, j.n_bits, j.n_bits_ext, j.n_cols, st,
+                         nullptr);
         if (!rc && d_roots) {
             uint64_t nt = pil2hip_tree_num_elements(1ull << j.n_bits_ext, j.arity);
             int e = p2_launch_copy_root(j.d_nodes + nt - 4, d_roots + 4 * order[k], st);
@@ -440,6 +464,42 @@ void pil2hip_batch_release(void)
     std::lock_guard<std::mutex> lk(g_batch_mu);
     for (auto &g : g_batch_graphs) hipGraphExecDestroy(g.exec);
     g_batch_graphs.clear();
+}
+
+uint64_t pil2hip_proof_levels(uint64_t nRows, uint64_t arity, uint64_t llv)
+{
+    if (arity < 2) return 0;
+    uint64_t cut = 1;
+    for (uint64_t k = 0; k < llv && cut <= nRows; k++) cut *= arity;
+    uint64_t n = nRows, levels = 0;
+    while (n > cut) {
+        n = (n + arity - 1) / arity;
+        levels++;
+    }
+    return levels;
+}
+
+uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uint64_t llv)
+{
+    return nCols + (arity - 1) * 4 * pil2hip_proof_levels(nRows, arity, llv);
+}
+
+int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t srcColStride, uint64_t nCols,
+                             const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+                             const uint64_t *d_idx, uint64_t nQueries, void *stream)
+{
+    int64_t rc = check_arity(arity);
+    if (rc) return rc;
+    if (nQueries == 0) return 0;
+    if (!d_out || !d_nodes || !d_idx || (nCols && !d_src)) return fail(-13, "null buffer");
+    if (llv > 8) return fail(-28, "last level verification > 8");
+    uint64_t cut = 1;
+    for (uint64_t k = 0; k < llv && cut <= nRows; k++) cut *= arity;
+    uint64_t pw = pil2hip_proof_words(nRows, nCols, arity, llv);
+    int e = p2_launch_group_proofs(d_src, srcColStride, nCols, d_nodes, nRows, arity, cut, d_idx, nQueries, pw, d_out,
+                                   (hipStream_t)stream);
+    if (e) return fail(-100 - e, "group proof launch failed");
+    return 0;
 }
 
 void pil2hip_timing_enable(bool on) { g_timing = on; }

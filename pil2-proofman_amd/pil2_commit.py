@@ -50,6 +50,19 @@ _SIGS = {
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_batch_release": (None, []),
+    "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
+    "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
+    "pil2hip_proof_words": (_u64, [_u64, _u64, _u64, _u64]),
+    "pil2hip_serve_create": (_vp, [_vp]),
+    "pil2hip_serve_submit": (_i64, [_vp, _vp]),
+    "pil2hip_serve_poll": (_i64, [_vp, _vp, _u64, ctypes.c_int32]),
+    "pil2hip_serve_cancel": (_i64, [_vp, _u64]),
+    "pil2hip_serve_open": (_i64, [_vp, _u64, _vp, _u64, _vp]),
+    "pil2hip_serve_release": (_i64, [_vp, _u64]),
+    "pil2hip_serve_stats": (_i64, [_vp, _vp]),
+    "pil2hip_serve_now_us": (ctypes.c_double, [_vp]),
+    "pil2hip_serve_job_bytes": (_u64, [_u64, _u64, _u64, _u64, _u64, _u64, ctypes.c_bool]),
+    "pil2hip_serve_destroy": (None, [_vp]),
     "pil2hip_get_root": (_i64, [_vp, _vp, _u64, _u64, _vp]),
     "pil2hip_fri_fold": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_fri_transpose": (_i64, [_vp, _vp, _u64, _u64, _vp]),
@@ -362,3 +375,130 @@ class CommitBatch:
 
 def batch_release():
     _lib.pil2hip_batch_release()
+
+
+def proof_words(n_rows, n_cols, arity, llv):
+    return int(_lib.pil2hip_proof_words(n_rows, n_cols, arity, llv))
+
+
+def group_proofs(lde, n_rows, n_cols, nodes, arity, llv, idx, stream=None):
+    """Device getGroupProof for many row indices (idx: device int64 tensor).
+    Returns a device tensor of len(idx) * proof_words(...) words."""
+    pw = proof_words(n_rows, n_cols, arity, llv)
+    out = empty(max(idx.numel() * pw, 1), lde.device)
+    _check(_lib.pil2hip_group_proofs(_ptr(out), _ptr(lde), n_rows, n_cols, _ptr(nodes), n_rows, arity, llv,
+                                     _ptr(idx), idx.numel(), _stream(stream)))
+    return out
+
+
+# -------------------------------------------------------------- serving --
+class Pil2ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", _u64), ("n_streams", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
+                ("arity", _u64), ("llv", _u64)]
+
+
+class Pil2ServeRequest(ctypes.Structure):
+    _fields_ = [("n_bits", _u64), ("n_bits_ext", _u64), ("n_cols", _u64), ("h_trace", _vp), ("d_trace", _vp),
+                ("seed", _u64), ("n_queries", _u64), ("h_queries", _vp), ("h_root", _vp), ("h_proofs", _vp),
+                ("tag", _u64), ("keep_resident", ctypes.c_uint32)]
+
+
+class Pil2ServeCompletion(ctypes.Structure):
+    _fields_ = [("job_id", _u64), ("tag", _u64), ("status", _i64), ("t_submit_us", ctypes.c_double),
+                ("t_admit_us", ctypes.c_double), ("t_done_us", ctypes.c_double), ("stream", ctypes.c_uint32)]
+
+
+class Pil2ServeStats(ctypes.Structure):
+    _fields_ = [(k, _u64) for k in ("submitted", "admitted", "completed", "failed", "cancelled", "queued",
+                                    "running", "resident_jobs", "bytes_in_use", "peak_bytes", "budget_bytes")] + \
+               [("peak_running", ctypes.c_uint32)]
+
+
+class ServeEngine:
+    """Commit server (pil2hip_serve_*): jobs are admitted by HBM budget as they
+    arrive and run concurrently; each completion carries the job's root and
+    group proofs (host numpy arrays owned by the job record)."""
+
+    def __init__(self, hbm_budget_bytes, n_streams=4, max_inflight=16, arity=3, llv=2):
+        cfg = Pil2ServeConfig(int(hbm_budget_bytes), n_streams, max_inflight, arity, llv)
+        self.arity, self.llv = arity, llv
+        self.h = _lib.pil2hip_serve_create(ctypes.byref(cfg))
+        if not self.h:
+            raise HipError("pil2hip_serve_create: " + _lib.pil2hip_last_error().decode())
+        self.records = {}
+
+    def submit(self, n_bits, n_bits_ext, n_cols, queries=(), seed=0, h_trace=None, d_trace=None, tag=0,
+               keep_resident=False):
+        """Queue a job; returns its id. h_trace: numpy uint64 (rows, cols); d_trace: device tensor."""
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.uint64))
+        pw = proof_words(1 << n_bits_ext, n_cols, self.arity, self.llv)
+        rec = {"root": np.zeros(4, dtype=np.uint64), "proofs": np.zeros(max(q.size * pw, 1), dtype=np.uint64),
+               "queries": q, "pw": pw, "n_bits": n_bits, "n_bits_ext": n_bits_ext, "n_cols": n_cols}
+        if h_trace is not None:
+            rec["h_trace"] = np.ascontiguousarray(h_trace, dtype=np.uint64)
+        rec["d_trace"] = d_trace
+        req = Pil2ServeRequest(n_bits, n_bits_ext, n_cols,
+                               rec["h_trace"].ctypes.data if h_trace is not None else None,
+                               d_trace.data_ptr() if d_trace is not None else None,
+                               seed, q.size, q.ctypes.data if q.size else None, rec["root"].ctypes.data,
+                               rec["proofs"].ctypes.data if q.size else None, tag, 1 if keep_resident else 0)
+        jid = _lib.pil2hip_serve_submit(self.h, ctypes.byref(req))
+        if jid < 0:
+            _check(jid)
+        self.records[jid] = rec
+        return jid
+
+    def poll(self, max_n=256, timeout_ms=100):
+        buf = (Pil2ServeCompletion * max_n)()
+        n = _lib.pil2hip_serve_poll(self.h, ctypes.cast(buf, _vp), max_n, timeout_ms)
+        if n < 0:
+            _check(n)
+        out = []
+        for c in buf[:n]:
+            rec = self.records.get(c.job_id, {})
+            rec.update(status=c.status, t_submit_us=c.t_submit_us, t_admit_us=c.t_admit_us, t_done_us=c.t_done_us,
+                       stream=c.stream, tag=c.tag, job_id=c.job_id)
+            out.append(rec)
+        return out
+
+    def cancel(self, job_id):
+        return _lib.pil2hip_serve_cancel(self.h, job_id)
+
+    def open(self, job_id, queries):
+        rec = self.records[job_id]
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.uint64))
+        out = np.zeros(max(q.size * rec["pw"], 1), dtype=np.uint64)
+        _check(_lib.pil2hip_serve_open(self.h, job_id, q.ctypes.data_as(_vp), q.size, out.ctypes.data_as(_vp)))
+        return out[:q.size * rec["pw"]]
+
+    def release(self, job_id):
+        _check(_lib.pil2hip_serve_release(self.h, job_id))
+
+    def forget(self, job_id):
+        """Release the job's resources in the engine and drop the host record."""
+        rc = _lib.pil2hip_serve_release(self.h, job_id)
+        self.records.pop(job_id, None)
+        return rc
+
+    def stats(self):
+        st = Pil2ServeStats()
+        _check(_lib.pil2hip_serve_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in Pil2ServeStats._fields_}
+
+    def now_us(self):
+        return _lib.pil2hip_serve_now_us(self.h)
+
+    def close(self):
+        if self.h:
+            _lib.pil2hip_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_job_bytes(n_bits, n_bits_ext, n_cols, arity, n_queries, llv, device_trace=False):
+    return int(_lib.pil2hip_serve_job_bytes(n_bits, n_bits_ext, n_cols, arity, n_queries, llv, device_trace))
