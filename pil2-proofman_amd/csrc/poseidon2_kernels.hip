@@ -15,6 +15,7 @@
 // One permutation per lane: the state lives in VGPRs, round constants are
 // wave-uniform and come from __constant__ memory through scalar loads.
 #include "gl_field.h"
+#include "p2_arith.h"
 #include "p2_constants.h"
 #include "poseidon2_kernels.h"
 
@@ -39,45 +40,107 @@ template <> struct P2Tab<16> {
     static __device__ __forceinline__ uint64_t d(int i) { return c_d16[i]; }
 };
 
-__device__ __forceinline__ uint64_t dbl(uint64_t a) { return gl_add(a, a); }
-
-// M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] applied with additions only.
-__device__ __forceinline__ void m4(uint64_t &x0, uint64_t &x1, uint64_t &x2, uint64_t &x3)
+// External layer M_E (poseidon2_goldilocks.hpp:186-231): M4 on every 4-block,
+// then every element adds the sum of its column over the blocks. Inputs are
+// 64-bit words, outputs lazy sums (carry count < 80: M4 rows sum to at most 16,
+// at most 4 blocks plus the element itself).
+// M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] as additions and shifts:
+// t0 = x0+x1, t1 = x2+x3, t2 = 2x1+t1, t3 = 2x3+t0, t4 = 4t1+t3, t5 = 4t0+t2,
+// (x0, x1, x2, x3) <- (t3+t5, t5, t2+t4, t4).
+template <int W>
+__device__ __forceinline__ void ext_layer_lz(const uint64_t *s, lz *t)
 {
-    uint64_t t0 = gl_add(x0, x1);
-    uint64_t t1 = gl_add(x2, x3);
-    uint64_t t2 = gl_add(dbl(x1), t1);
-    uint64_t t3 = gl_add(dbl(x3), t0);
-    uint64_t t4 = gl_add(dbl(dbl(t1)), t3);
-    uint64_t t5 = gl_add(dbl(dbl(t0)), t2);
-    uint64_t t6 = gl_add(t3, t5);
-    uint64_t t7 = gl_add(t2, t4);
-    x0 = t6; x1 = t5; x2 = t7; x3 = t4;
+    constexpr int NB = W / 4, N2 = 2 * NB;
+    lz x[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) x[i] = lz_of(s[i]);
+    lz a[N2], b[N2], t01[N2], t23[N2], t45[N2], t67[N2];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        a[2 * j] = x[4 * j]; b[2 * j] = x[4 * j + 1];
+        a[2 * j + 1] = x[4 * j + 2]; b[2 * j + 1] = x[4 * j + 3];
+    }
+    lz_add_all<N2>(t01, a, b);  // t0, t1
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        a[2 * j] = lz_shl<1>(x[4 * j + 1]); b[2 * j] = t01[2 * j + 1];
+        a[2 * j + 1] = lz_shl<1>(x[4 * j + 3]); b[2 * j + 1] = t01[2 * j];
+    }
+    lz_add_all<N2>(t23, a, b);  // t2, t3
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        a[2 * j] = lz_shl<2>(t01[2 * j + 1]); b[2 * j] = t23[2 * j + 1];
+        a[2 * j + 1] = lz_shl<2>(t01[2 * j]); b[2 * j + 1] = t23[2 * j];
+    }
+    lz_add_all<N2>(t45, a, b);  // t4, t5
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        a[2 * j] = t23[2 * j + 1]; b[2 * j] = t45[2 * j + 1];
+        a[2 * j + 1] = t23[2 * j]; b[2 * j + 1] = t45[2 * j];
+    }
+    lz_add_all<N2>(t67, a, b);  // t6, t7
+    lz y[W];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        y[4 * j] = t67[2 * j]; y[4 * j + 1] = t45[2 * j + 1];
+        y[4 * j + 2] = t67[2 * j + 1]; y[4 * j + 3] = t45[2 * j];
+    }
+    // column sums over the blocks
+    lz col[4];
+    lz_add_all<4>(col, y, y + 4);
+#pragma unroll
+    for (int j = 2; j < NB; j++) lz_add_all<4>(col, col, y + 4 * j);
+    lz cc[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) cc[i] = col[i & 3];
+    lz_add_all<W>(t, y, cc);
+}
+
+// x^7 (any 64-bit word in, congruent word out)
+__device__ __forceinline__ uint64_t pow7(uint64_t x)
+{
+    uint64_t x2 = p2_mul(x, x);
+    uint64_t x3 = p2_mul(x2, x);
+    uint64_t x4 = p2_mul(x2, x2);
+    return p2_mul(x3, x4);
+}
+
+// Full round on a lazy state t: S-box(t + rc) on every element, then M_E.
+template <int W>
+__device__ __forceinline__ void full_round(lz *t, int rc_base)
+{
+    lz rc[W], u[W];
+    uint64_t s[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) rc[i] = lz_of(P2Tab<W>::rc(rc_base + i));
+    lz_add_all<W>(u, t, rc);
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = pow7(lz_fold(u[i]));
+    ext_layer_lz<W>(s, t);
+}
+
+// Sum of N lazy values (pairwise tree, additions interleaved per level).
+template <int N>
+__device__ __forceinline__ lz sum_tree(const lz *v)
+{
+    if constexpr (N == 1) {
+        return v[0];
+    } else {
+        constexpr int H = N / 2;
+        lz r[H + (N & 1)];
+        lz_add_all<H>(r, v, v + H);
+        if constexpr (N & 1) r[H] = v[N - 1];
+        return sum_tree<H + (N & 1)>(r);
+    }
 }
 
 template <int W>
-__device__ __forceinline__ void ext_layer(uint64_t *s)
+__device__ __forceinline__ lz sum_lz(const uint64_t *s)
 {
+    lz v[W];
 #pragma unroll
-    for (int i = 0; i < W; i += 4) m4(s[i], s[i + 1], s[i + 2], s[i + 3]);
-    uint64_t c[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        c[k] = s[k];
-#pragma unroll
-        for (int i = 4 + k; i < W; i += 4) c[k] = gl_add(c[k], s[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < W; i++) s[i] = gl_add(s[i], c[i & 3]);
-}
-
-// x^7; intermediate powers only feed multiplications, so they stay non-canonical
-__device__ __forceinline__ uint64_t pow7(uint64_t x)
-{
-    uint64_t x2 = gl_mul_nc(x, x);
-    uint64_t x3 = gl_mul_nc(x2, x);
-    uint64_t x4 = gl_mul_nc(x2, x2);
-    return gl_mul(x3, x4);
+    for (int i = 0; i < W; i++) v[i] = lz_of(s[i]);
+    return sum_tree<W>(v);
 }
 
 template <int W>
@@ -85,28 +148,36 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
 {
     // Round loops stay rolled: the unrolled permutation would not fit the
     // instruction cache, and per-round constants are wave-uniform scalar loads.
-    ext_layer<W>(s);
+    lz t[W];
+    ext_layer_lz<W>(s, t);
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < 4; r++) full_round<W>(t, r * W);
+    // partial rounds: S-box on element 0, then s <- diag(d) s + sum(s)
+    {
+        lz rc0 = lz_of(P2Tab<W>::rc(4 * W)), u;
+        lz_add_n<1>(&u, &t[0], &rc0);
+        s[0] = pow7(lz_fold(u));
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(r * W + i)));
-        ext_layer<W>(s);
+        for (int i = 1; i < W; i++) s[i] = lz_fold(t[i]);
     }
 #pragma unroll 1
     for (int r = 0; r < 22; r++) {
-        s[0] = pow7(gl_add(s[0], P2Tab<W>::rc(4 * W + r)));
-        uint64_t sum = s[0];
+        if (r > 0) {
+            lz rc0 = lz_of(P2Tab<W>::rc(4 * W + r)), u, x0 = lz_of(s[0]);
+            lz_add_n<1>(&u, &x0, &rc0);
+            s[0] = pow7(lz_fold(u));
+        }
+        uint64_t sum = lz_fold(sum_lz<W>(s));
+        uint32_t s_lo = (uint32_t)sum, s_hi = (uint32_t)(sum >> 32);
 #pragma unroll
-        for (int i = 1; i < W; i++) sum = gl_add(sum, s[i]);
-#pragma unroll
-        for (int i = 0; i < W; i++) s[i] = gl_add(gl_mul(s[i], P2Tab<W>::d(i)), sum);
+        for (int i = 0; i < W; i++) s[i] = p2_mul_add(s[i], P2Tab<W>::d(i), s_lo, s_hi);
     }
+#pragma unroll
+    for (int i = 0; i < W; i++) t[i] = lz_of(s[i]);
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < 4; r++) full_round<W>(t, 4 * W + 22 + r * W);
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = pow7(gl_add(s[i], P2Tab<W>::rc(4 * W + 22 + r * W + i)));
-        ext_layer<W>(s);
-    }
+    for (int i = 0; i < W; i++) s[i] = lz_fold(t[i]);
 }
 
 // ------------------------------------------------------------ leaves ----
@@ -134,15 +205,15 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_leaves(const uint64_t *__rest
         }
         if (c0 + RATE <= ncols) {
 #pragma unroll
-            for (int i = 0; i < RATE; i++) s[i] = gl_canon(row[(c0 + i) * cs]);
+            for (int i = 0; i < RATE; i++) s[i] = row[(c0 + i) * cs];
         } else {
 #pragma unroll
-            for (int i = 0; i < RATE; i++) s[i] = (c0 + i < ncols) ? gl_canon(row[(c0 + i) * cs]) : 0;
+            for (int i = 0; i < RATE; i++) s[i] = (c0 + i < ncols) ? row[(c0 + i) * cs] : 0;
         }
         p2_permute<W>(s);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) out[r * 4 + i] = s[i];
+    for (int i = 0; i < 4; i++) out[r * 4 + i] = gl_canon(s[i]);
 }
 
 // ------------------------------------------------------------- nodes ----
@@ -170,7 +241,7 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_nodes(uint64_t *__restrict__ 
     p2_permute<W>(s);
     uint64_t *o = tree + in_off + (pending + pad) * 4 + i * 4;
 #pragma unroll
-    for (int k = 0; k < 4; k++) o[k] = s[k];
+    for (int k = 0; k < 4; k++) o[k] = gl_canon(s[k]);
 }
 
 // Single permutation over `count` states stored consecutively (W words each).
@@ -181,10 +252,10 @@ __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
     if (i >= count) return;
     uint64_t s[W];
 #pragma unroll
-    for (int k = 0; k < W; k++) s[k] = gl_canon(st[i * W + k]);
+    for (int k = 0; k < W; k++) s[k] = st[i * W + k];
     p2_permute<W>(s);
 #pragma unroll
-    for (int k = 0; k < W; k++) st[i * W + k] = s[k];
+    for (int k = 0; k < W; k++) st[i * W + k] = gl_canon(s[k]);
 }
 
 // ------------------------------------------------------------ openings ----

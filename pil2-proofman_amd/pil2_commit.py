@@ -274,6 +274,16 @@ class FRI:
     """Mirror of FRI<Goldilocks::Element> (fri.hpp) fold and per-layer commit."""
 
     @staticmethod
+    def steps(n_bits_ext, folding=3, final_degree=5):
+        """FRI step schedule (setup/pil2-stark/src/types/stark_struct.rs:162-166):
+        nBits of every layer, from the LDE domain down to the final polynomial."""
+        out, b = [n_bits_ext], n_bits_ext
+        while b > final_degree + 1:
+            b = b - folding if b > folding + final_degree else final_degree
+            out.append(b)
+        return out
+
+    @staticmethod
     def fold(step, pol, challenge, n_bits_ext, prev_bits, current_bits, out=None, stream=None):
         """fri.hpp:32-99. pol: device tensor of 2^prev_bits cubic values (3 words each).
         Folds in place unless `out` is given; returns the tensor holding the result."""

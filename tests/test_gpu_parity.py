@@ -238,6 +238,13 @@ def test_fri_fold_all_layers(pc, oracle, nbe):
     chal = datagen.challenges(len(steps), 0x5EED0004 + 1)
     d = pc.to_device(pol)
     host = pol.copy()
+    # step 0 commits the LDE-domain polynomial before any fold (gen_proof order)
+    w0 = 3 << (steps[0] - steps[1])
+    tree0 = pc.MerkleTreeGL(3, 2, False, 1 << steps[1], w0)
+    root0 = pc.FRI.merkelize(tree0, d, steps[0], steps[1])
+    aux0 = oracle.fri_transpose(host, 1 << steps[0], steps[1])
+    assert (root0 == oracle.merkletree(aux0.reshape(1 << steps[1], w0), 3)[-4:]).all(), "root 0"
+    assert pc.FRI.steps(nbe) == steps
     for s in range(1, len(steps)):
         prev, cur = steps[s - 1], steps[s]
         print("fri step", s, prev, "->", cur, flush=True)
