@@ -91,14 +91,17 @@ __device__ __forceinline__ uint64_t gl_reduce_words(uint32_t x0, uint32_t x1, ui
         "s_andn2_b64 %[mp], %[ca], vcc\n\t"
         "s_andn2_b64 vcc, vcc, %[ca]\n\t"
         "v_cndmask_b32_e64 %[t0], 0, -1, %[mp]\n\t"
+        "v_cndmask_b32_e64 %[t1], 0, -1, vcc\n\t"
         "v_cndmask_b32_e64 %[t0], %[t0], 1, vcc\n\t"
-        "v_cndmask_b32_e64 %[t1], 0, -1, vcc"
+        // (r1:r0) += (t1:t0); the correction cannot wrap (see header)
+        "v_add_co_u32_e32 %[r0], vcc, %[r0], %[t0]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e32 %[r1], vcc, %[r1], %[t1], vcc"
         : [r0] "=&v"(r0), [r1] "=&v"(r1), [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [ca] "=&s"(ca),
           [mp] "=&s"(mp)
         : [x0] "v"(x0), [x1] "v"(x1), [x2] "v"(x2), [x3] "v"(x3)
         : "vcc", "scc");
-    // (r1:r0) + (t1:t0): the correction cannot wrap (see header)
-    return gl_pack(r0, r1) + gl_pack(t0, t1);
+    return gl_pack(r0, r1);
 }
 
 // Any 64-bit word -> canonical: r >= p  <=>  r + (2^32 - 1) carries out of 64 bits.

@@ -105,17 +105,49 @@ __device__ __forceinline__ uint64_t pow7(uint64_t x)
     return p2_mul(x3, x4);
 }
 
+// x^7 on three lanes, each product interleaved across the lanes
+__device__ __forceinline__ void pow7_3(const uint64_t *x, uint64_t *out)
+{
+    uint64_t x2[3], x3[3], x4[3];
+    p2_mul3(x, x, x2);
+    p2_mul3(x2, x, x3);
+    p2_mul3(x2, x2, x4);
+    p2_mul3(x3, x4, out);
+}
+
+// s[i] = fold(t[i]) for all lanes, three at a time
+template <int W>
+__device__ __forceinline__ void fold_all(const lz *t, uint64_t *s)
+{
+    constexpr int G = W / 3 * 3;
+#pragma unroll
+    for (int i = 0; i < G; i += 3) lz_fold3(t + i, s + i);
+#pragma unroll
+    for (int i = G; i < W; i++) s[i] = lz_fold(t[i]);
+}
+
 // Full round on a lazy state t: S-box(t + rc) on every element, then M_E.
 template <int W>
 __device__ __forceinline__ void full_round(lz *t, int rc_base)
 {
-    lz rc[W], u[W];
-    uint64_t s[W];
+    constexpr int G = W / 3 * 3;  // lanes handled three at a time
+    uint64_t rcv[W], x[W], s[W];
+    lz u[W];
 #pragma unroll
-    for (int i = 0; i < W; i++) rc[i] = lz_of(P2Tab<W>::rc(rc_base + i));
-    lz_add_all<W>(u, t, rc);
+    for (int i = 0; i < W; i++) rcv[i] = P2Tab<W>::rc(rc_base + i);
 #pragma unroll
-    for (int i = 0; i < W; i++) s[i] = pow7(lz_fold(u[i]));
+    for (int i = 0; i < G; i += 3) lz_add_rc3(u + i, t + i, rcv + i);
+    if constexpr (G < W) {
+        lz rc[W - G];
+#pragma unroll
+        for (int i = 0; i < W - G; i++) rc[i] = lz_of(rcv[G + i]);
+        lz_add_all<W - G>(u + G, t + G, rc);
+    }
+    fold_all<W>(u, x);
+#pragma unroll
+    for (int i = 0; i < G; i += 3) pow7_3(x + i, s + i);
+#pragma unroll
+    for (int i = G; i < W; i++) s[i] = pow7(x[i]);
     ext_layer_lz<W>(s, t);
 }
 
@@ -148,36 +180,33 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
 {
     // Round loops stay rolled: the unrolled permutation would not fit the
     // instruction cache, and per-round constants are wave-uniform scalar loads.
+    constexpr int G = W / 3 * 3;
     lz t[W];
     ext_layer_lz<W>(s, t);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) full_round<W>(t, r * W);
+    fold_all<W>(t, s);
     // partial rounds: S-box on element 0, then s <- diag(d) s + sum(s)
-    {
-        lz rc0 = lz_of(P2Tab<W>::rc(4 * W)), u;
-        lz_add_n<1>(&u, &t[0], &rc0);
-        s[0] = pow7(lz_fold(u));
-#pragma unroll
-        for (int i = 1; i < W; i++) s[i] = lz_fold(t[i]);
-    }
 #pragma unroll 1
     for (int r = 0; r < 22; r++) {
-        if (r > 0) {
-            lz rc0 = lz_of(P2Tab<W>::rc(4 * W + r)), u, x0 = lz_of(s[0]);
-            lz_add_n<1>(&u, &x0, &rc0);
-            s[0] = pow7(lz_fold(u));
-        }
+        lz rc0 = lz_of(P2Tab<W>::rc(4 * W + r)), u, x0 = lz_of(s[0]);
+        lz_add_n<1>(&u, &x0, &rc0);
+        s[0] = pow7(lz_fold(u));
         uint64_t sum = lz_fold(sum_lz<W>(s));
         uint32_t s_lo = (uint32_t)sum, s_hi = (uint32_t)(sum >> 32);
+        uint64_t d[W];
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = p2_mul_add(s[i], P2Tab<W>::d(i), s_lo, s_hi);
+        for (int i = 0; i < W; i++) d[i] = P2Tab<W>::d(i);
+#pragma unroll
+        for (int i = 0; i < G; i += 3) p2_mul_add3(s + i, d + i, s_lo, s_hi, s + i);
+#pragma unroll
+        for (int i = G; i < W; i++) s[i] = p2_mul_add(s[i], d[i], s_lo, s_hi);
     }
 #pragma unroll
     for (int i = 0; i < W; i++) t[i] = lz_of(s[i]);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) full_round<W>(t, 4 * W + 22 + r * W);
-#pragma unroll
-    for (int i = 0; i < W; i++) s[i] = lz_fold(t[i]);
+    fold_all<W>(t, s);
 }
 
 // ------------------------------------------------------------ leaves ----

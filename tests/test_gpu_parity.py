@@ -95,6 +95,41 @@ def test_permute_random_batch(pc, oracle):
             assert (got[i] == oracle.permute(states[i * w:(i + 1) * w])).all()
 
 
+# words at the carry and borrow boundaries of the non-canonical arithmetic,
+# including values >= p (any 64-bit word is a valid field representative)
+EDGE_WORDS = [0, 1, 2, 2**31, 2**32 - 1, 2**32, 2**32 + 1, 2**63 - 1, 2**63, P - 2**32, P - 2, P - 1, P, P + 1,
+              P + 2**32 - 2, 2**64 - 2**33, 2**64 - 2, 2**64 - 1]
+
+
+def test_permute_edge_words(pc, oracle):
+    """States drawn from boundary words (and all-equal states of each) through
+    the permutation, every state checked against the oracle."""
+    rng = np.random.default_rng(2024)
+    for w in (8, 12, 16):
+        n_rand = 600
+        states = np.array(EDGE_WORDS, dtype=np.uint64)[rng.integers(0, len(EDGE_WORDS), (n_rand, w))]
+        const = np.repeat(np.array(EDGE_WORDS, dtype=np.uint64)[:, None], w, axis=1)
+        allst = np.concatenate([states, const]).astype(np.uint64)
+        d = pc.to_device(allst.reshape(-1))
+        pc.p2_permute(d, w)
+        _sync()
+        got = pc.to_host(d).reshape(-1, w)
+        for i in range(allst.shape[0]):
+            assert (got[i] == oracle.permute(allst[i])).all(), (w, i, [hex(int(v)) for v in allst[i]])
+
+
+def test_merkle_edge_words(pc, oracle):
+    """Rows of boundary words (non-canonical included) hashed into a tree."""
+    rng = np.random.default_rng(7)
+    rows, cols = 1 << 9, 21
+    src = np.array(EDGE_WORDS, dtype=np.uint64)[rng.integers(0, len(EDGE_WORDS), (rows, cols))]
+    for arity in (2, 3, 4):
+        t = pc.MerkleTreeGL(arity, 0, True, rows, cols, source=pc.to_device(src))
+        t.merkelize()
+        _sync()
+        assert (pc.to_host(t.nodes) == oracle.merkletree(src, arity)).all(), arity
+
+
 # ------------------------------------------------------------------- LDE --
 LDE_SHAPES = [(0, 1, 4), (1, 1, 3), (1, 3, 2), (3, 5, 5), (4, 5, 8), (6, 8, 12), (10, 12, 1), (12, 13, 8),
               (12, 12, 7), (13, 14, 32), (11, 13, 3), (14, 16, 5), (15, 16, 4), (16, 17, 32), (17, 19, 3),
