@@ -6,18 +6,12 @@
 // reference may hold non-canonical words (>= p) and compares through toU64();
 // canonical outputs therefore compare equal bit for bit after the reference's toU64.
 //
-// Device code (gfx950) works on 32-bit halves with VOP carry chains:
-//   product  a*b = p00 + 2^32 (p01 + p10) + 2^64 p11 (four v_mad_u64_u32), combined
-//            into 128 bits x = (x3:x2:x1:x0) by two interleaved carry chains;
-//   reduce   2^64 = 2^32 - 1 and 2^96 = -1 (mod p) give
-//              x = (x1:x0) + x2*2^32 - x2 - x3 = A - B' + cA*(2^32 - 1)
-//            with A = ((x1 + x2) mod 2^32 : x0), cA its carry, B' = x2 + x3 (33 bits);
-//            a borrow of A - B' wraps by 2^64 = 2^32 - 1, so the value is
-//            (A - B') + (cA - borrow)*(2^32 - 1), and neither correction wraps again
-//            (cA = 1 forces A < p - 1; borrow = 1 forces A - B' + 2^64 > 2^64 - 2^33).
-// gfx950 needs two wait states between a VALU write of an SGPR/VCC lane mask and
-// a VALU read of it (carry-in or select); the inline sequences below insert them
-// (s_nop) or fill them with independent instructions.
+// Device code (gfx950): products are chains of v_mad_u64_u32 whose addends carry
+// the partial sums, reduced with one more v_mad_u64_u32 and a short correction
+// (gl_prod_mac / gl_prod_reduce*, below); additions and subtractions are 32-bit
+// carry chains. gfx950 needs two wait states between a VALU write of an SGPR/VCC
+// lane mask and a VALU read of it (carry-in or select); the inline sequences
+// below insert them (s_nop) or fill them with independent instructions.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -43,64 +37,114 @@ __host__ __device__ __forceinline__ uint64_t gl_reduce128(uint64_t hi, uint64_t 
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// 128-bit product of two 64-bit words, as 32-bit words x0..x3.
-__device__ __forceinline__ void gl_mul_words(uint64_t a, uint64_t b, uint32_t &x0, uint32_t &x1, uint32_t &x2,
-                                             uint32_t &x3)
+// ---- multiply-accumulate products (no carry chains) ----------------------
+// a*b for 64-bit a = a1:a0, b = b1:b0 as a chain of v_mad_u64_u32 whose 64-bit
+// addends absorb the previous partial product, so no step can overflow
+// ((2^32-1)^2 + 2*(2^32-1) = 2^64 - 1):
+//   p = a0*b0 + s_lo                  x0 = p mod 2^32
+//   t = a0*b1 + (p >> 32) + s_hi
+//   u = a1*b0 + (t mod 2^32)          x1 = u mod 2^32
+//   (x3:x2) = a1*b1 + (t >> 32) + (u >> 32)
+// (s = s_hi:s_lo is an optional addend of the whole product.) Reduction then
+// uses 2^64 = 2^32 - 1 and 2^96 = -1 (mod p):
+//   v = (x1:x0) + x2*(2^32 - 1) - x3,
+// computed as r = mad(x2, 2^32 - 1, x1:x0) with carry-out c, then r - x3 with
+// borrow b, then + (c - b)*(2^32 - 1): c = 1 forces r <= 2^64 - 2^33 and b = 1
+// with c = 0 forces r - x3 + 2^64 > 2^64 - 2^32, so the correction never wraps.
+struct gl_prod {
+    uint64_t lo;     // (x1:x0)
+    uint32_t x2, x3;
+};
+
+__device__ __forceinline__ gl_prod gl_prod_mac(uint64_t a, uint64_t b, uint32_t s_lo, uint32_t s_hi)
 {
     uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    uint64_t p00 = (uint64_t)a0 * b0, p01 = (uint64_t)a0 * b1, p10 = (uint64_t)a1 * b0, p11 = (uint64_t)a1 * b1;
-    uint32_t y1, y2, y3;
-    uint64_t c2;
-    // chain 1 (vcc):  (y1,y2,y3) = [h0,l3,h3] + [l1,h1,0]
-    // chain 2 (c2):   (y1,y2,y3) += [l2,h2,0]
-    asm("v_add_co_u32_e32 %[y1], vcc, %[h0], %[l1]\n\t"
-        "v_add_co_u32_e64 %[y1], %[c2], %[y1], %[l2]\n\t"
-        "s_nop 0\n\t"
-        "v_addc_co_u32_e32 %[y2], vcc, %[l3], %[h1], vcc\n\t"
-        "v_addc_co_u32_e64 %[y2], %[c2], %[y2], %[h2], %[c2]\n\t"
-        "s_nop 0\n\t"
-        "v_addc_co_u32_e32 %[y3], vcc, 0, %[h3], vcc\n\t"
-        "v_addc_co_u32_e64 %[y3], %[c2], %[y3], 0, %[c2]"
-        : [y1] "=&v"(y1), [y2] "=&v"(y2), [y3] "=&v"(y3), [c2] "=&s"(c2)
-        : [h0] "v"((uint32_t)(p00 >> 32)), [l1] "v"((uint32_t)p01), [h1] "v"((uint32_t)(p01 >> 32)),
-          [l2] "v"((uint32_t)p10), [h2] "v"((uint32_t)(p10 >> 32)), [l3] "v"((uint32_t)p11),
-          [h3] "v"((uint32_t)(p11 >> 32))
-        : "vcc", "scc");
-    x0 = (uint32_t)p00;
-    x1 = y1;
-    x2 = y2;
-    x3 = y3;
+    uint64_t p = (uint64_t)a0 * b0 + s_lo;
+    uint64_t t = (uint64_t)a0 * b1 + ((p >> 32) + (uint64_t)s_hi);
+    uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
+    uint64_t h = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
+    gl_prod r;
+    r.lo = gl_pack((uint32_t)p, (uint32_t)u);
+    r.x2 = (uint32_t)h;
+    r.x3 = (uint32_t)(h >> 32);
+    return r;
 }
 
-#endif
-
-#if defined(__HIP_DEVICE_COMPILE__)
-// 128-bit value -> 64-bit word congruent mod p (not necessarily canonical).
-__device__ __forceinline__ uint64_t gl_reduce_words(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3)
+// Reduce three products in lockstep (three instructions between dependent
+// carry-mask writes and reads: no s_nop).
+__device__ __forceinline__ void gl_prod_reduce3(const gl_prod *x, uint64_t *out)
 {
-    uint32_t r0, r1, t0, t1, t2;
-    uint64_t ca, mp;
-    asm("v_add_co_u32_e64 %[t0], %[ca], %[x1], %[x2]\n\t"
-        "v_add_co_u32_e32 %[t1], vcc, %[x2], %[x3]\n\t"
+    uint64_t r[3], c[3];
+    asm("v_mad_u64_u32 %0, %3, %6, -1, %9\n\t"
+        "v_mad_u64_u32 %1, %4, %7, -1, %10\n\t"
+        "v_mad_u64_u32 %2, %5, %8, -1, %11"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&s"(c[0]), "=&s"(c[1]), "=&s"(c[2])
+        : "v"(x[0].x2), "v"(x[1].x2), "v"(x[2].x2), "v"(x[0].lo), "v"(x[1].lo), "v"(x[2].lo));
+    uint32_t r0[3], r1[3], t0[3], t1[3];
+    uint64_t b[3], m[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        r0[k] = (uint32_t)r[k];
+        r1[k] = (uint32_t)(r[k] >> 32);
+    }
+    asm("v_sub_co_u32_e64 %[r0a], %[ba], %[r0a], %[x3a]\n\t"
+        "v_sub_co_u32_e64 %[r0b], %[bb], %[r0b], %[x3b]\n\t"
+        "v_sub_co_u32_e64 %[r0c], %[bc], %[r0c], %[x3c]\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[ba], %[r1a], 0, %[ba]\n\t"
+        "v_subb_co_u32_e64 %[r1b], %[bb], %[r1b], 0, %[bb]\n\t"
+        "v_subb_co_u32_e64 %[r1c], %[bc], %[r1c], 0, %[bc]\n\t"
+        "s_andn2_b64 %[ma], %[ca], %[ba]\n\t"
+        "s_andn2_b64 %[mb], %[cb], %[bb]\n\t"
+        "s_andn2_b64 %[mc], %[cc], %[bc]\n\t"
+        "s_andn2_b64 %[ba], %[ba], %[ca]\n\t"
+        "s_andn2_b64 %[bb], %[bb], %[cb]\n\t"
+        "s_andn2_b64 %[bc], %[bc], %[cc]\n\t"
+        "v_cndmask_b32_e64 %[t0a], 0, -1, %[ma]\n\t"
+        "v_cndmask_b32_e64 %[t0b], 0, -1, %[mb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], 0, -1, %[mc]\n\t"
+        "v_cndmask_b32_e64 %[t1a], 0, -1, %[ba]\n\t"
+        "v_cndmask_b32_e64 %[t1b], 0, -1, %[bb]\n\t"
+        "v_cndmask_b32_e64 %[t1c], 0, -1, %[bc]\n\t"
+        "v_cndmask_b32_e64 %[t0a], %[t0a], 1, %[ba]\n\t"
+        "v_cndmask_b32_e64 %[t0b], %[t0b], 1, %[bb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], %[t0c], 1, %[bc]\n\t"
+        "v_add_co_u32_e64 %[r0a], %[ca], %[r0a], %[t0a]\n\t"
+        "v_add_co_u32_e64 %[r0b], %[cb], %[r0b], %[t0b]\n\t"
+        "v_add_co_u32_e64 %[r0c], %[cc], %[r0c], %[t0c]\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[ca], %[r1a], %[t1a], %[ca]\n\t"
+        "v_addc_co_u32_e64 %[r1b], %[cb], %[r1b], %[t1b], %[cb]\n\t"
+        "v_addc_co_u32_e64 %[r1c], %[cc], %[r1c], %[t1c], %[cc]"
+        : [r0a] "+v"(r0[0]), [r0b] "+v"(r0[1]), [r0c] "+v"(r0[2]), [r1a] "+v"(r1[0]), [r1b] "+v"(r1[1]),
+          [r1c] "+v"(r1[2]), [t0a] "=&v"(t0[0]), [t0b] "=&v"(t0[1]), [t0c] "=&v"(t0[2]), [t1a] "=&v"(t1[0]),
+          [t1b] "=&v"(t1[1]), [t1c] "=&v"(t1[2]), [ba] "=&s"(b[0]), [bb] "=&s"(b[1]), [bc] "=&s"(b[2]),
+          [ma] "=&s"(m[0]), [mb] "=&s"(m[1]), [mc] "=&s"(m[2]), [ca] "+s"(c[0]), [cb] "+s"(c[1]), [cc] "+s"(c[2])
+        : [x3a] "v"(x[0].x3), [x3b] "v"(x[1].x3), [x3c] "v"(x[2].x3)
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = gl_pack(r0[k], r1[k]);
+}
+
+// Single product reduction (hazard gaps filled with s_nop).
+__device__ __forceinline__ uint64_t gl_prod_reduce1(gl_prod x)
+{
+    uint64_t r, c, b, m;
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(x.x2), "v"(x.lo));
+    uint32_t r0 = (uint32_t)r, r1 = (uint32_t)(r >> 32), t0, t1;
+    asm("v_sub_co_u32_e64 %[r0], %[b], %[r0], %[x3]\n\t"
         "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %[t2], 0, 1, vcc\n\t"
-        "v_sub_co_u32_e32 %[r0], vcc, %[x0], %[t1]\n\t"
+        "v_subb_co_u32_e64 %[r1], %[b], %[r1], 0, %[b]\n\t"
         "s_nop 1\n\t"
-        "v_subb_co_u32_e32 %[r1], vcc, %[t0], %[t2], vcc\n\t"
+        "s_andn2_b64 %[m], %[c], %[b]\n\t"
+        "s_andn2_b64 %[b], %[b], %[c]\n\t"
+        "v_cndmask_b32_e64 %[t0], 0, -1, %[m]\n\t"
+        "v_cndmask_b32_e64 %[t1], 0, -1, %[b]\n\t"
+        "v_cndmask_b32_e64 %[t0], %[t0], 1, %[b]\n\t"
+        "v_add_co_u32_e64 %[r0], %[c], %[r0], %[t0]\n\t"
         "s_nop 1\n\t"
-        "s_andn2_b64 %[mp], %[ca], vcc\n\t"
-        "s_andn2_b64 vcc, vcc, %[ca]\n\t"
-        "v_cndmask_b32_e64 %[t0], 0, -1, %[mp]\n\t"
-        "v_cndmask_b32_e64 %[t1], 0, -1, vcc\n\t"
-        "v_cndmask_b32_e64 %[t0], %[t0], 1, vcc\n\t"
-        // (r1:r0) += (t1:t0); the correction cannot wrap (see header)
-        "v_add_co_u32_e32 %[r0], vcc, %[r0], %[t0]\n\t"
-        "s_nop 1\n\t"
-        "v_addc_co_u32_e32 %[r1], vcc, %[r1], %[t1], vcc"
-        : [r0] "=&v"(r0), [r1] "=&v"(r1), [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [ca] "=&s"(ca),
-          [mp] "=&s"(mp)
-        : [x0] "v"(x0), [x1] "v"(x1), [x2] "v"(x2), [x3] "v"(x3)
-        : "vcc", "scc");
+        "v_addc_co_u32_e64 %[r1], %[c], %[r1], %[t1], %[c]"
+        : [r0] "+v"(r0), [r1] "+v"(r1), [t0] "=&v"(t0), [t1] "=&v"(t1), [b] "=&s"(b), [m] "=&s"(m), [c] "+s"(c)
+        : [x3] "v"(x.x3)
+        : "scc");
     return gl_pack(r0, r1);
 }
 
@@ -183,9 +227,7 @@ __host__ __device__ __forceinline__ uint64_t gl_neg(uint64_t a) { return a ? GL_
 __host__ __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t x0, x1, x2, x3;
-    gl_mul_words(a, b, x0, x1, x2, x3);
-    return gl_canon_dev(gl_reduce_words(x0, x1, x2, x3));
+    return gl_canon_dev(gl_prod_reduce1(gl_prod_mac(a, b, 0, 0)));
 #else
     unsigned __int128 x = (unsigned __int128)a * b;
     return gl_reduce128((uint64_t)(x >> 64), (uint64_t)x);
@@ -197,9 +239,7 @@ __host__ __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b)
 __host__ __device__ __forceinline__ uint64_t gl_mul_nc(uint64_t a, uint64_t b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t x0, x1, x2, x3;
-    gl_mul_words(a, b, x0, x1, x2, x3);
-    return gl_reduce_words(x0, x1, x2, x3);
+    return gl_prod_reduce1(gl_prod_mac(a, b, 0, 0));
 #else
     return gl_mul(a, b);
 #endif
@@ -209,7 +249,11 @@ __host__ __device__ __forceinline__ uint64_t gl_mul_nc(uint64_t a, uint64_t b)
 __host__ __device__ __forceinline__ uint64_t gl_reduce128_canon(uint64_t lo, uint64_t hi)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return gl_canon_dev(gl_reduce_words((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)));
+    gl_prod x;
+    x.lo = lo;
+    x.x2 = (uint32_t)hi;
+    x.x3 = (uint32_t)(hi >> 32);
+    return gl_canon_dev(gl_prod_reduce1(x));
 #else
     return gl_reduce128(hi, lo);
 #endif
