@@ -148,6 +148,602 @@ __device__ __forceinline__ uint64_t gl_prod_reduce1(gl_prod x)
     return gl_pack(r0, r1);
 }
 
+
+
+template <int N> __device__ __forceinline__ void gl_add_n(uint64_t *out, const uint64_t *a, const uint64_t *b);
+template <int N> __device__ __forceinline__ void gl_sub_n(uint64_t *out, const uint64_t *a, const uint64_t *b);
+template <int N> __device__ __forceinline__ void gl_prod_reduce_canon_n(uint64_t *out, const gl_prod *x);
+
+// ---- lane-interleaved canonical operations (N = 1..4 independent lanes) ----
+// Every instruction of a lane is N instructions after the previous one of the
+// same lane, so for N >= 3 carry-mask reads need no s_nop; N = 1, 2 pad with
+// s_nop to the two wait states gfx950 needs.
+
+template <> __device__ __forceinline__ void gl_add_n<1>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[1], s1[1], t0[1], t1[1];
+    uint64_t K[1], M[1];
+    asm(
+        "v_add_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], -1, %[s0a]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], 0, %[s1a], %[Ma]\n\t"
+        "s_nop 1\n\t"
+        "s_or_b64 %[Ka], %[Ka], %[Ma]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 1; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_sub_n<1>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[1], s1[1], t0[1], t1[1];
+    uint64_t K[1], M[1];
+    asm(
+        "v_sub_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], 1, %[s0a]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], -1, %[s1a], %[Ma]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 1; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_prod_reduce_canon_n<1>(uint64_t *out, const gl_prod *x)
+{
+    uint64_t r[1], c[1], B[1], P[1], K[1];
+    uint32_t r0[1], r1[1], t0[1], t1[1];
+#pragma unroll
+    for (int k = 0; k < 1; k++) {
+        asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r[k]), "=&s"(c[k]) : "v"(x[k].x2), "v"(x[k].lo));
+        r0[k] = (uint32_t)r[k];
+        r1[k] = (uint32_t)(r[k] >> 32);
+    }
+    asm(
+        "v_sub_co_u32_e64 %[r0a], %[Ba], %[r0a], %[x3a]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[Ba], %[r1a], 0, %[Ba]\n\t"
+        "s_nop 1\n\t"
+        "s_andn2_b64 %[Pa], %[Ca], %[Ba]\n\t"
+        "s_nop 1\n\t"
+        "s_andn2_b64 %[Ba], %[Ba], %[Ca]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[t0a], 0, -1, %[Pa]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[t1a], 0, -1, %[Ba]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[t0a], %[t0a], 1, %[Ba]\n\t"
+        "s_nop 1\n\t"
+        "v_add_co_u32_e64 %[r0a], %[Ca], %[r0a], %[t0a]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[Ca], %[r1a], %[t1a], %[Ca]"
+        : [r0a] "+v"(r0[0]), [r1a] "+v"(r1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ba] "=&s"(B[0]),
+          [Pa] "=&s"(P[0]), [Ca] "+s"(c[0])
+        : [x3a] "v"(x[0].x3)
+        : "scc");
+    uint32_t *s0 = r0, *s1 = r1;
+    asm(
+        "v_add_co_u32_e64 %[t0a], %[Ka], -1, %[s0a]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ka], 0, %[s1a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]"
+        : [s0a] "+v"(s0[0]), [s1a] "+v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0])
+        :
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 1; k++) out[k] = gl_pack(r0[k], r1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_add_n<2>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[2], s1[2], t0[2], t1[2];
+    uint64_t K[2], M[2];
+    asm(
+        "v_add_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_add_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], -1, %[s0b]\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], 0, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], 0, %[s1b], %[Mb]\n\t"
+        "s_nop 0\n\t"
+        "s_or_b64 %[Ka], %[Ka], %[Ma]\n\t"
+        "s_or_b64 %[Kb], %[Kb], %[Mb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 2; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_sub_n<2>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[2], s1[2], t0[2], t1[2];
+    uint64_t K[2], M[2];
+    asm(
+        "v_sub_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_sub_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_subb_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], 1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], 1, %[s0b]\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], -1, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], -1, %[s1b], %[Mb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 2; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_prod_reduce_canon_n<2>(uint64_t *out, const gl_prod *x)
+{
+    uint64_t r[2], c[2], B[2], P[2], K[2];
+    uint32_t r0[2], r1[2], t0[2], t1[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r[k]), "=&s"(c[k]) : "v"(x[k].x2), "v"(x[k].lo));
+        r0[k] = (uint32_t)r[k];
+        r1[k] = (uint32_t)(r[k] >> 32);
+    }
+    asm(
+        "v_sub_co_u32_e64 %[r0a], %[Ba], %[r0a], %[x3a]\n\t"
+        "v_sub_co_u32_e64 %[r0b], %[Bb], %[r0b], %[x3b]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[Ba], %[r1a], 0, %[Ba]\n\t"
+        "v_subb_co_u32_e64 %[r1b], %[Bb], %[r1b], 0, %[Bb]\n\t"
+        "s_nop 0\n\t"
+        "s_andn2_b64 %[Pa], %[Ca], %[Ba]\n\t"
+        "s_andn2_b64 %[Pb], %[Cb], %[Bb]\n\t"
+        "s_nop 0\n\t"
+        "s_andn2_b64 %[Ba], %[Ba], %[Ca]\n\t"
+        "s_andn2_b64 %[Bb], %[Bb], %[Cb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[t0a], 0, -1, %[Pa]\n\t"
+        "v_cndmask_b32_e64 %[t0b], 0, -1, %[Pb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[t1a], 0, -1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t1b], 0, -1, %[Bb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[t0a], %[t0a], 1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t0b], %[t0b], 1, %[Bb]\n\t"
+        "s_nop 0\n\t"
+        "v_add_co_u32_e64 %[r0a], %[Ca], %[r0a], %[t0a]\n\t"
+        "v_add_co_u32_e64 %[r0b], %[Cb], %[r0b], %[t0b]\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[Ca], %[r1a], %[t1a], %[Ca]\n\t"
+        "v_addc_co_u32_e64 %[r1b], %[Cb], %[r1b], %[t1b], %[Cb]"
+        : [r0a] "+v"(r0[0]), [r1a] "+v"(r1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ba] "=&s"(B[0]),
+          [Pa] "=&s"(P[0]), [Ca] "+s"(c[0]), [r0b] "+v"(r0[1]), [r1b] "+v"(r1[1]), [t0b] "=&v"(t0[1]),
+          [t1b] "=&v"(t1[1]), [Bb] "=&s"(B[1]), [Pb] "=&s"(P[1]), [Cb] "+s"(c[1])
+        : [x3a] "v"(x[0].x3), [x3b] "v"(x[1].x3)
+        : "scc");
+    uint32_t *s0 = r0, *s1 = r1;
+    asm(
+        "v_add_co_u32_e64 %[t0a], %[Ka], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Kb], -1, %[s0b]\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ka], 0, %[s1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Kb], 0, %[s1b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]"
+        : [s0a] "+v"(s0[0]), [s1a] "+v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [s0b] "+v"(s0[1]), [s1b] "+v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]), [Kb] "=&s"(K[1])
+        :
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 2; k++) out[k] = gl_pack(r0[k], r1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_add_n<3>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[3], s1[3], t0[3], t1[3];
+    uint64_t K[3], M[3];
+    asm(
+        "v_add_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_add_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "v_add_co_u32_e64 %[s0c], %[Kc], %[a0c], %[b0c]\n\t"
+        "v_addc_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "v_addc_co_u32_e64 %[s1c], %[Kc], %[a1c], %[b1c], %[Kc]\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], -1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Mc], -1, %[s0c]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], 0, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], 0, %[s1b], %[Mb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Mc], 0, %[s1c], %[Mc]\n\t"
+        "s_or_b64 %[Ka], %[Ka], %[Ma]\n\t"
+        "s_or_b64 %[Kb], %[Kb], %[Mb]\n\t"
+        "s_or_b64 %[Kc], %[Kc], %[Mc]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1]), [s0c] "=&v"(s0[2]), [s1c] "=&v"(s1[2]), [t0c] "=&v"(t0[2]),
+          [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2]), [Mc] "=&s"(M[2])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32)), [a0c] "v"((uint32_t)a[2]),
+          [a1c] "v"((uint32_t)(a[2] >> 32)), [b0c] "v"((uint32_t)b[2]), [b1c] "v"((uint32_t)(b[2] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_sub_n<3>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[3], s1[3], t0[3], t1[3];
+    uint64_t K[3], M[3];
+    asm(
+        "v_sub_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_sub_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "v_sub_co_u32_e64 %[s0c], %[Kc], %[a0c], %[b0c]\n\t"
+        "v_subb_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_subb_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "v_subb_co_u32_e64 %[s1c], %[Kc], %[a1c], %[b1c], %[Kc]\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], 1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], 1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Mc], 1, %[s0c]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], -1, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], -1, %[s1b], %[Mb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Mc], -1, %[s1c], %[Mc]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1]), [s0c] "=&v"(s0[2]), [s1c] "=&v"(s1[2]), [t0c] "=&v"(t0[2]),
+          [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2]), [Mc] "=&s"(M[2])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32)), [a0c] "v"((uint32_t)a[2]),
+          [a1c] "v"((uint32_t)(a[2] >> 32)), [b0c] "v"((uint32_t)b[2]), [b1c] "v"((uint32_t)(b[2] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_prod_reduce_canon_n<3>(uint64_t *out, const gl_prod *x)
+{
+    uint64_t r[3], c[3], B[3], P[3], K[3];
+    uint32_t r0[3], r1[3], t0[3], t1[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r[k]), "=&s"(c[k]) : "v"(x[k].x2), "v"(x[k].lo));
+        r0[k] = (uint32_t)r[k];
+        r1[k] = (uint32_t)(r[k] >> 32);
+    }
+    asm(
+        "v_sub_co_u32_e64 %[r0a], %[Ba], %[r0a], %[x3a]\n\t"
+        "v_sub_co_u32_e64 %[r0b], %[Bb], %[r0b], %[x3b]\n\t"
+        "v_sub_co_u32_e64 %[r0c], %[Bc], %[r0c], %[x3c]\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[Ba], %[r1a], 0, %[Ba]\n\t"
+        "v_subb_co_u32_e64 %[r1b], %[Bb], %[r1b], 0, %[Bb]\n\t"
+        "v_subb_co_u32_e64 %[r1c], %[Bc], %[r1c], 0, %[Bc]\n\t"
+        "s_andn2_b64 %[Pa], %[Ca], %[Ba]\n\t"
+        "s_andn2_b64 %[Pb], %[Cb], %[Bb]\n\t"
+        "s_andn2_b64 %[Pc], %[Cc], %[Bc]\n\t"
+        "s_andn2_b64 %[Ba], %[Ba], %[Ca]\n\t"
+        "s_andn2_b64 %[Bb], %[Bb], %[Cb]\n\t"
+        "s_andn2_b64 %[Bc], %[Bc], %[Cc]\n\t"
+        "v_cndmask_b32_e64 %[t0a], 0, -1, %[Pa]\n\t"
+        "v_cndmask_b32_e64 %[t0b], 0, -1, %[Pb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], 0, -1, %[Pc]\n\t"
+        "v_cndmask_b32_e64 %[t1a], 0, -1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t1b], 0, -1, %[Bb]\n\t"
+        "v_cndmask_b32_e64 %[t1c], 0, -1, %[Bc]\n\t"
+        "v_cndmask_b32_e64 %[t0a], %[t0a], 1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t0b], %[t0b], 1, %[Bb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], %[t0c], 1, %[Bc]\n\t"
+        "v_add_co_u32_e64 %[r0a], %[Ca], %[r0a], %[t0a]\n\t"
+        "v_add_co_u32_e64 %[r0b], %[Cb], %[r0b], %[t0b]\n\t"
+        "v_add_co_u32_e64 %[r0c], %[Cc], %[r0c], %[t0c]\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[Ca], %[r1a], %[t1a], %[Ca]\n\t"
+        "v_addc_co_u32_e64 %[r1b], %[Cb], %[r1b], %[t1b], %[Cb]\n\t"
+        "v_addc_co_u32_e64 %[r1c], %[Cc], %[r1c], %[t1c], %[Cc]"
+        : [r0a] "+v"(r0[0]), [r1a] "+v"(r1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ba] "=&s"(B[0]),
+          [Pa] "=&s"(P[0]), [Ca] "+s"(c[0]), [r0b] "+v"(r0[1]), [r1b] "+v"(r1[1]), [t0b] "=&v"(t0[1]),
+          [t1b] "=&v"(t1[1]), [Bb] "=&s"(B[1]), [Pb] "=&s"(P[1]), [Cb] "+s"(c[1]), [r0c] "+v"(r0[2]),
+          [r1c] "+v"(r1[2]), [t0c] "=&v"(t0[2]), [t1c] "=&v"(t1[2]), [Bc] "=&s"(B[2]), [Pc] "=&s"(P[2]),
+          [Cc] "+s"(c[2])
+        : [x3a] "v"(x[0].x3), [x3b] "v"(x[1].x3), [x3c] "v"(x[2].x3)
+        : "scc");
+    uint32_t *s0 = r0, *s1 = r1;
+    asm(
+        "v_add_co_u32_e64 %[t0a], %[Ka], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Kb], -1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Kc], -1, %[s0c]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ka], 0, %[s1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Kb], 0, %[s1b], %[Kb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Kc], 0, %[s1c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]"
+        : [s0a] "+v"(s0[0]), [s1a] "+v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [s0b] "+v"(s0[1]), [s1b] "+v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]), [Kb] "=&s"(K[1]),
+          [s0c] "+v"(s0[2]), [s1c] "+v"(s1[2]), [t0c] "=&v"(t0[2]), [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2])
+        :
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = gl_pack(r0[k], r1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_add_n<4>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[4], s1[4], t0[4], t1[4];
+    uint64_t K[4], M[4];
+    asm(
+        "v_add_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_add_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "v_add_co_u32_e64 %[s0c], %[Kc], %[a0c], %[b0c]\n\t"
+        "v_add_co_u32_e64 %[s0d], %[Kd], %[a0d], %[b0d]\n\t"
+        "v_addc_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "v_addc_co_u32_e64 %[s1c], %[Kc], %[a1c], %[b1c], %[Kc]\n\t"
+        "v_addc_co_u32_e64 %[s1d], %[Kd], %[a1d], %[b1d], %[Kd]\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], -1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Mc], -1, %[s0c]\n\t"
+        "v_add_co_u32_e64 %[t0d], %[Md], -1, %[s0d]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], 0, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], 0, %[s1b], %[Mb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Mc], 0, %[s1c], %[Mc]\n\t"
+        "v_addc_co_u32_e64 %[t1d], %[Md], 0, %[s1d], %[Md]\n\t"
+        "s_or_b64 %[Ka], %[Ka], %[Ma]\n\t"
+        "s_or_b64 %[Kb], %[Kb], %[Mb]\n\t"
+        "s_or_b64 %[Kc], %[Kc], %[Mc]\n\t"
+        "s_or_b64 %[Kd], %[Kd], %[Md]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s0d], %[s0d], %[t0d], %[Kd]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1d], %[s1d], %[t1d], %[Kd]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1]), [s0c] "=&v"(s0[2]), [s1c] "=&v"(s1[2]), [t0c] "=&v"(t0[2]),
+          [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2]), [Mc] "=&s"(M[2]), [s0d] "=&v"(s0[3]), [s1d] "=&v"(s1[3]),
+          [t0d] "=&v"(t0[3]), [t1d] "=&v"(t1[3]), [Kd] "=&s"(K[3]), [Md] "=&s"(M[3])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32)), [a0c] "v"((uint32_t)a[2]),
+          [a1c] "v"((uint32_t)(a[2] >> 32)), [b0c] "v"((uint32_t)b[2]), [b1c] "v"((uint32_t)(b[2] >> 32)),
+          [a0d] "v"((uint32_t)a[3]), [a1d] "v"((uint32_t)(a[3] >> 32)), [b0d] "v"((uint32_t)b[3]),
+          [b1d] "v"((uint32_t)(b[3] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_sub_n<4>(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    uint32_t s0[4], s1[4], t0[4], t1[4];
+    uint64_t K[4], M[4];
+    asm(
+        "v_sub_co_u32_e64 %[s0a], %[Ka], %[a0a], %[b0a]\n\t"
+        "v_sub_co_u32_e64 %[s0b], %[Kb], %[a0b], %[b0b]\n\t"
+        "v_sub_co_u32_e64 %[s0c], %[Kc], %[a0c], %[b0c]\n\t"
+        "v_sub_co_u32_e64 %[s0d], %[Kd], %[a0d], %[b0d]\n\t"
+        "v_subb_co_u32_e64 %[s1a], %[Ka], %[a1a], %[b1a], %[Ka]\n\t"
+        "v_subb_co_u32_e64 %[s1b], %[Kb], %[a1b], %[b1b], %[Kb]\n\t"
+        "v_subb_co_u32_e64 %[s1c], %[Kc], %[a1c], %[b1c], %[Kc]\n\t"
+        "v_subb_co_u32_e64 %[s1d], %[Kd], %[a1d], %[b1d], %[Kd]\n\t"
+        "v_add_co_u32_e64 %[t0a], %[Ma], 1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Mb], 1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Mc], 1, %[s0c]\n\t"
+        "v_add_co_u32_e64 %[t0d], %[Md], 1, %[s0d]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ma], -1, %[s1a], %[Ma]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Mb], -1, %[s1b], %[Mb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Mc], -1, %[s1c], %[Mc]\n\t"
+        "v_addc_co_u32_e64 %[t1d], %[Md], -1, %[s1d], %[Md]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s0d], %[s0d], %[t0d], %[Kd]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1d], %[s1d], %[t1d], %[Kd]"
+        : [s0a] "=&v"(s0[0]), [s1a] "=&v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [Ma] "=&s"(M[0]), [s0b] "=&v"(s0[1]), [s1b] "=&v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]),
+          [Kb] "=&s"(K[1]), [Mb] "=&s"(M[1]), [s0c] "=&v"(s0[2]), [s1c] "=&v"(s1[2]), [t0c] "=&v"(t0[2]),
+          [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2]), [Mc] "=&s"(M[2]), [s0d] "=&v"(s0[3]), [s1d] "=&v"(s1[3]),
+          [t0d] "=&v"(t0[3]), [t1d] "=&v"(t1[3]), [Kd] "=&s"(K[3]), [Md] "=&s"(M[3])
+        : [a0a] "v"((uint32_t)a[0]), [a1a] "v"((uint32_t)(a[0] >> 32)), [b0a] "v"((uint32_t)b[0]),
+          [b1a] "v"((uint32_t)(b[0] >> 32)), [a0b] "v"((uint32_t)a[1]), [a1b] "v"((uint32_t)(a[1] >> 32)),
+          [b0b] "v"((uint32_t)b[1]), [b1b] "v"((uint32_t)(b[1] >> 32)), [a0c] "v"((uint32_t)a[2]),
+          [a1c] "v"((uint32_t)(a[2] >> 32)), [b0c] "v"((uint32_t)b[2]), [b1c] "v"((uint32_t)(b[2] >> 32)),
+          [a0d] "v"((uint32_t)a[3]), [a1d] "v"((uint32_t)(a[3] >> 32)), [b0d] "v"((uint32_t)b[3]),
+          [b1d] "v"((uint32_t)(b[3] >> 32))
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = gl_pack(s0[k], s1[k]);
+}
+
+template <> __device__ __forceinline__ void gl_prod_reduce_canon_n<4>(uint64_t *out, const gl_prod *x)
+{
+    uint64_t r[4], c[4], B[4], P[4], K[4];
+    uint32_t r0[4], r1[4], t0[4], t1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r[k]), "=&s"(c[k]) : "v"(x[k].x2), "v"(x[k].lo));
+        r0[k] = (uint32_t)r[k];
+        r1[k] = (uint32_t)(r[k] >> 32);
+    }
+    asm(
+        "v_sub_co_u32_e64 %[r0a], %[Ba], %[r0a], %[x3a]\n\t"
+        "v_sub_co_u32_e64 %[r0b], %[Bb], %[r0b], %[x3b]\n\t"
+        "v_sub_co_u32_e64 %[r0c], %[Bc], %[r0c], %[x3c]\n\t"
+        "v_sub_co_u32_e64 %[r0d], %[Bd], %[r0d], %[x3d]\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[Ba], %[r1a], 0, %[Ba]\n\t"
+        "v_subb_co_u32_e64 %[r1b], %[Bb], %[r1b], 0, %[Bb]\n\t"
+        "v_subb_co_u32_e64 %[r1c], %[Bc], %[r1c], 0, %[Bc]\n\t"
+        "v_subb_co_u32_e64 %[r1d], %[Bd], %[r1d], 0, %[Bd]\n\t"
+        "s_andn2_b64 %[Pa], %[Ca], %[Ba]\n\t"
+        "s_andn2_b64 %[Pb], %[Cb], %[Bb]\n\t"
+        "s_andn2_b64 %[Pc], %[Cc], %[Bc]\n\t"
+        "s_andn2_b64 %[Pd], %[Cd], %[Bd]\n\t"
+        "s_andn2_b64 %[Ba], %[Ba], %[Ca]\n\t"
+        "s_andn2_b64 %[Bb], %[Bb], %[Cb]\n\t"
+        "s_andn2_b64 %[Bc], %[Bc], %[Cc]\n\t"
+        "s_andn2_b64 %[Bd], %[Bd], %[Cd]\n\t"
+        "v_cndmask_b32_e64 %[t0a], 0, -1, %[Pa]\n\t"
+        "v_cndmask_b32_e64 %[t0b], 0, -1, %[Pb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], 0, -1, %[Pc]\n\t"
+        "v_cndmask_b32_e64 %[t0d], 0, -1, %[Pd]\n\t"
+        "v_cndmask_b32_e64 %[t1a], 0, -1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t1b], 0, -1, %[Bb]\n\t"
+        "v_cndmask_b32_e64 %[t1c], 0, -1, %[Bc]\n\t"
+        "v_cndmask_b32_e64 %[t1d], 0, -1, %[Bd]\n\t"
+        "v_cndmask_b32_e64 %[t0a], %[t0a], 1, %[Ba]\n\t"
+        "v_cndmask_b32_e64 %[t0b], %[t0b], 1, %[Bb]\n\t"
+        "v_cndmask_b32_e64 %[t0c], %[t0c], 1, %[Bc]\n\t"
+        "v_cndmask_b32_e64 %[t0d], %[t0d], 1, %[Bd]\n\t"
+        "v_add_co_u32_e64 %[r0a], %[Ca], %[r0a], %[t0a]\n\t"
+        "v_add_co_u32_e64 %[r0b], %[Cb], %[r0b], %[t0b]\n\t"
+        "v_add_co_u32_e64 %[r0c], %[Cc], %[r0c], %[t0c]\n\t"
+        "v_add_co_u32_e64 %[r0d], %[Cd], %[r0d], %[t0d]\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[Ca], %[r1a], %[t1a], %[Ca]\n\t"
+        "v_addc_co_u32_e64 %[r1b], %[Cb], %[r1b], %[t1b], %[Cb]\n\t"
+        "v_addc_co_u32_e64 %[r1c], %[Cc], %[r1c], %[t1c], %[Cc]\n\t"
+        "v_addc_co_u32_e64 %[r1d], %[Cd], %[r1d], %[t1d], %[Cd]"
+        : [r0a] "+v"(r0[0]), [r1a] "+v"(r1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ba] "=&s"(B[0]),
+          [Pa] "=&s"(P[0]), [Ca] "+s"(c[0]), [r0b] "+v"(r0[1]), [r1b] "+v"(r1[1]), [t0b] "=&v"(t0[1]),
+          [t1b] "=&v"(t1[1]), [Bb] "=&s"(B[1]), [Pb] "=&s"(P[1]), [Cb] "+s"(c[1]), [r0c] "+v"(r0[2]),
+          [r1c] "+v"(r1[2]), [t0c] "=&v"(t0[2]), [t1c] "=&v"(t1[2]), [Bc] "=&s"(B[2]), [Pc] "=&s"(P[2]),
+          [Cc] "+s"(c[2]), [r0d] "+v"(r0[3]), [r1d] "+v"(r1[3]), [t0d] "=&v"(t0[3]), [t1d] "=&v"(t1[3]),
+          [Bd] "=&s"(B[3]), [Pd] "=&s"(P[3]), [Cd] "+s"(c[3])
+        : [x3a] "v"(x[0].x3), [x3b] "v"(x[1].x3), [x3c] "v"(x[2].x3), [x3d] "v"(x[3].x3)
+        : "scc");
+    uint32_t *s0 = r0, *s1 = r1;
+    asm(
+        "v_add_co_u32_e64 %[t0a], %[Ka], -1, %[s0a]\n\t"
+        "v_add_co_u32_e64 %[t0b], %[Kb], -1, %[s0b]\n\t"
+        "v_add_co_u32_e64 %[t0c], %[Kc], -1, %[s0c]\n\t"
+        "v_add_co_u32_e64 %[t0d], %[Kd], -1, %[s0d]\n\t"
+        "v_addc_co_u32_e64 %[t1a], %[Ka], 0, %[s1a], %[Ka]\n\t"
+        "v_addc_co_u32_e64 %[t1b], %[Kb], 0, %[s1b], %[Kb]\n\t"
+        "v_addc_co_u32_e64 %[t1c], %[Kc], 0, %[s1c], %[Kc]\n\t"
+        "v_addc_co_u32_e64 %[t1d], %[Kd], 0, %[s1d], %[Kd]\n\t"
+        "v_cndmask_b32_e64 %[s0a], %[s0a], %[t0a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s0b], %[s0b], %[t0b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s0c], %[s0c], %[t0c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s0d], %[s0d], %[t0d], %[Kd]\n\t"
+        "v_cndmask_b32_e64 %[s1a], %[s1a], %[t1a], %[Ka]\n\t"
+        "v_cndmask_b32_e64 %[s1b], %[s1b], %[t1b], %[Kb]\n\t"
+        "v_cndmask_b32_e64 %[s1c], %[s1c], %[t1c], %[Kc]\n\t"
+        "v_cndmask_b32_e64 %[s1d], %[s1d], %[t1d], %[Kd]"
+        : [s0a] "+v"(s0[0]), [s1a] "+v"(s1[0]), [t0a] "=&v"(t0[0]), [t1a] "=&v"(t1[0]), [Ka] "=&s"(K[0]),
+          [s0b] "+v"(s0[1]), [s1b] "+v"(s1[1]), [t0b] "=&v"(t0[1]), [t1b] "=&v"(t1[1]), [Kb] "=&s"(K[1]),
+          [s0c] "+v"(s0[2]), [s1c] "+v"(s1[2]), [t0c] "=&v"(t0[2]), [t1c] "=&v"(t1[2]), [Kc] "=&s"(K[2]),
+          [s0d] "+v"(s0[3]), [s1d] "+v"(s1[3]), [t0d] "=&v"(t0[3]), [t1d] "=&v"(t1[3]), [Kd] "=&s"(K[3])
+        :
+        : "scc");
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = gl_pack(r0[k], r1[k]);
+}
+
+// out[k] = a[k] op b[k] for k < N, in interleaved groups of GL_LANES lanes
+#ifndef GL_LANES
+#define GL_LANES 3
+#endif
+template <int N>
+__device__ __forceinline__ void gl_add_v(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    constexpr int G = GL_LANES;
+#pragma unroll
+    for (int k = 0; k + G <= N; k += G) gl_add_n<G>(out + k, a + k, b + k);
+    if constexpr (N % G) gl_add_n<N % G>(out + N / G * G, a + N / G * G, b + N / G * G);
+}
+template <int N>
+__device__ __forceinline__ void gl_sub_v(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    constexpr int G = GL_LANES;
+#pragma unroll
+    for (int k = 0; k + G <= N; k += G) gl_sub_n<G>(out + k, a + k, b + k);
+    if constexpr (N % G) gl_sub_n<N % G>(out + N / G * G, a + N / G * G, b + N / G * G);
+}
+// canonical products out[k] = a[k] * b[k]
+template <int N>
+__device__ __forceinline__ void gl_mul_v(uint64_t *out, const uint64_t *a, const uint64_t *b)
+{
+    constexpr int G = GL_LANES;
+    gl_prod x[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) x[k] = gl_prod_mac(a[k], b[k], 0, 0);
+#pragma unroll
+    for (int k = 0; k + G <= N; k += G) gl_prod_reduce_canon_n<G>(out + k, x + k);
+    if constexpr (N % G) gl_prod_reduce_canon_n<N % G>(out + N / G * G, x + N / G * G);
+}
+
 // Any 64-bit word -> canonical: r >= p  <=>  r + (2^32 - 1) carries out of 64 bits.
 __device__ __forceinline__ uint64_t gl_canon_dev(uint64_t r)
 {
@@ -163,6 +759,10 @@ __device__ __forceinline__ uint64_t gl_canon_dev(uint64_t r)
         : "vcc", "scc");
     return gl_pack(lo, hi);
 }
+#else  // host pass: declarations only (kernels are compiled for the device)
+template <int N> __device__ void gl_add_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
+template <int N> __device__ void gl_sub_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
+template <int N> __device__ void gl_mul_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
 #endif
 
 // a, b canonical -> canonical a + b

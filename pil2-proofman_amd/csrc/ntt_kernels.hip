@@ -95,46 +95,113 @@ __global__ __launch_bounds__(256) void k_transpose_cm2rm(const uint64_t *__restr
 // R-point DIF (natural in -> bit-reversed out) on v[0..R), roots w_R^k = w_16^(k*16/R).
 template <int R> struct Log2 { static constexpr int v = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1; };
 
+// Butterfly k of a stage with half-size H (R/2 butterflies) pairs values
+// b + jj and b + jj + H, b = (k / H) * 2H, jj = k mod H; its twiddle is
+// w_{2H}^jj = w_16^(jj * 8 / H). The m-th butterfly with jj != 0:
+template <int H>
+__device__ constexpr int tw_k(int m)
+{
+    return (m / (H - 1)) * H + m % (H - 1) + 1;
+}
+
+// One DIF stage on R values in registers: (u, x) -> (u + x, (u - x) * w); the
+// additions, subtractions and non-trivial products each run lane-interleaved.
+template <int R, bool INV, int ST>
+__device__ __forceinline__ void dif_stage(uint64_t *v)
+{
+    constexpr int H = 1 << ST, NB = R / 2;
+    uint64_t u[NB], x[NB], sm[NB], df[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        u[k] = v[(k / H) * 2 * H + k % H];
+        x[k] = v[(k / H) * 2 * H + k % H + H];
+    }
+    gl_add_v<NB>(sm, u, x);
+    gl_sub_v<NB>(df, u, x);
+    if constexpr (H > 1) {
+        constexpr int NM = NB / H * (H - 1);
+        uint64_t mi[NM], mw[NM], mo[NM];
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            mi[m] = df[tw_k<H>(m)];
+            mw[m] = INV ? c_w16i[tw_k<H>(m) % H * 8 / H] : c_w16f[tw_k<H>(m) % H * 8 / H];
+        }
+        gl_mul_v<NM>(mo, mi, mw);
+#pragma unroll
+        for (int m = 0; m < NM; m++) df[tw_k<H>(m)] = mo[m];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        v[(k / H) * 2 * H + k % H] = sm[k];
+        v[(k / H) * 2 * H + k % H + H] = df[k];
+    }
+}
+
+// One DIT stage: (u, x) -> (u + x*w, u - x*w).
+template <int R, bool INV, int ST>
+__device__ __forceinline__ void dit_stage(uint64_t *v)
+{
+    constexpr int H = 1 << ST, NB = R / 2;
+    uint64_t u[NB], x[NB], sm[NB], df[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        u[k] = v[(k / H) * 2 * H + k % H];
+        x[k] = v[(k / H) * 2 * H + k % H + H];
+    }
+    if constexpr (H > 1) {
+        constexpr int NM = NB / H * (H - 1);
+        uint64_t mi[NM], mw[NM], mo[NM];
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            mi[m] = x[tw_k<H>(m)];
+            mw[m] = INV ? c_w16i[tw_k<H>(m) % H * 8 / H] : c_w16f[tw_k<H>(m) % H * 8 / H];
+        }
+        gl_mul_v<NM>(mo, mi, mw);
+#pragma unroll
+        for (int m = 0; m < NM; m++) x[tw_k<H>(m)] = mo[m];
+    }
+    gl_add_v<NB>(sm, u, x);
+    gl_sub_v<NB>(df, u, x);
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        v[(k / H) * 2 * H + k % H] = sm[k];
+        v[(k / H) * 2 * H + k % H + H] = df[k];
+    }
+}
+
+template <int R, bool INV, int ST>
+__device__ __forceinline__ void dif_stages(uint64_t *v)
+{
+    dif_stage<R, INV, ST>(v);
+    if constexpr (ST > 0) dif_stages<R, INV, ST - 1>(v);
+}
+
+template <int R, bool INV, int ST>
+__device__ __forceinline__ void dit_stages(uint64_t *v)
+{
+    dit_stage<R, INV, ST>(v);
+    if constexpr (ST + 1 < Log2<R>::v) dit_stages<R, INV, ST + 1>(v);
+}
+
+// R-point DIF (natural in -> bit-reversed out), R <= 16
 template <int R, bool INV>
 __device__ __forceinline__ void dft_dif(uint64_t *v)
 {
-    // stage loops use a linear counter so they unroll completely (a shifted
-    // counter left v[] dynamically indexed)
-#pragma unroll
-    for (int st = Log2<R>::v - 1; st >= 0; st--) {
-        const int h = 1 << st;
-#pragma unroll
-        for (int b = 0; b < R; b += 2 * h) {
-#pragma unroll
-            for (int jj = 0; jj < h; jj++) {
-                uint64_t u = v[b + jj], x = v[b + jj + h];
-                v[b + jj] = gl_add(u, x);
-                uint64_t d = gl_sub(u, x);
-                // w_{2h}^jj = w_16^(jj * 8 / h)
-                v[b + jj + h] = jj ? gl_mul(d, INV ? c_w16i[jj * 8 / h] : c_w16f[jj * 8 / h]) : d;
-            }
-        }
-    }
+    dif_stages<R, INV, Log2<R>::v - 1>(v);
 }
 
 // R-point DIT (bit-reversed in -> natural out)
 template <int R, bool INV>
 __device__ __forceinline__ void dft_dit(uint64_t *v)
 {
-#pragma unroll
-    for (int st = 0; st < Log2<R>::v; st++) {
-        const int h = 1 << st;
-#pragma unroll
-        for (int b = 0; b < R; b += 2 * h) {
-#pragma unroll
-            for (int jj = 0; jj < h; jj++) {
-                uint64_t u = v[b + jj], x = v[b + jj + h];
-                if (jj) x = gl_mul(x, INV ? c_w16i[jj * 8 / h] : c_w16f[jj * 8 / h]);
-                v[b + jj] = gl_add(u, x);
-                v[b + jj + h] = gl_sub(u, x);
-            }
-        }
-    }
+    dit_stages<R, INV, 0>(v);
+}
+
+// v[j] *= w[j] for j = 1 .. R-1, lane-interleaved
+template <int R>
+__device__ __forceinline__ void twiddle_rest(uint64_t *v, const uint64_t *w)
+{
+    if constexpr (R > 1) gl_mul_v<R - 1>(v + 1, v + 1, w + 1);
 }
 
 // Element position of value j of group gidx in a round over index bits [q, q+r)
@@ -181,11 +248,10 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
         dft_dif<R, true>(v);
         if (p.o_lo) {
+            uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
-                v[j] = gl_mul(v[j], tb.t14i[y]);
-            }
+            for (int j = 1; j < R; j++) w[j] = tb.t14i[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            twiddle_rest<R>(v, w);
         }
 #pragma unroll
         for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
@@ -208,11 +274,10 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
         if (p.o_lo) {
+            uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
-                v[j] = gl_mul(v[j], tb.t14f[y]);
-            }
+            for (int j = 1; j < R; j++) w[j] = tb.t14f[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            twiddle_rest<R>(v, w);
         }
         dft_dit<R, false>(v);
 #pragma unroll
@@ -251,13 +316,25 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
         }
         __syncthreads();
     }
+    // store: the tile holds 16 values per thread; four at a time, twiddles interleaved
     uint32_t shift_n = 24 - (L + log_n2);
-    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
-        uint32_t m = x >> lt, ii = x & ((1u << lt) - 1);
-        uint32_t e = (uint32_t)((i0 + ii) * bitrev32(m, L));
-        uint64_t v = s[lds_addr(x)];
-        if (e) v = gl_mul(v, tw_pow(tb.twl_i, tb.twh_i, e << shift_n));
-        col[(uint64_t)m * n2 + i0 + ii] = v;
+    for (uint32_t x0 = threadIdx.x; x0 < tile; x0 += 4 * blockDim.x) {
+        uint64_t v[4], lo[4], hi[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t x = x0 + k * blockDim.x;
+            uint32_t e = (uint32_t)((i0 + (x & ((1u << lt) - 1))) * bitrev32(x >> lt, L)) << shift_n;
+            v[k] = s[lds_addr(x)];
+            lo[k] = tb.twl_i[e & 4095];
+            hi[k] = tb.twh_i[e >> 12];
+        }
+        gl_mul_v<4>(lo, lo, hi);
+        gl_mul_v<4>(v, v, lo);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t x = x0 + k * blockDim.x;
+            col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))] = v[k];
+        }
     }
 }
 
@@ -308,22 +385,26 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
         for (int j = 0; j < R; j++) v[j] = s[lds_addr(round_m(p, j, q, LR))];
         dft_dif<R, true>(v);
         if (p.o_lo) {
+            uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
-                v[j] = gl_mul(v[j], tb.t14i[y]);
+            for (int j = 1; j < R; j++) w[j] = tb.t14i[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            twiddle_rest<R>(v, w);
+        }
+        if (last) {  // coefficient k of the line times 7^e / N, e = bitrev(m*n2 + k)
+            uint64_t lo[R], hi[R], ni[R];
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                uint32_t e = bitrev32((m << L2) | round_m(p, j, q, LR), log_n);
+                lo[j] = tb.shl[e & 4095];
+                hi[j] = tb.shh[e >> 12];
+                ni[j] = n_inv;
             }
+            gl_mul_v<R>(lo, lo, hi);
+            gl_mul_v<R>(v, v, ni);
+            gl_mul_v<R>(v, v, lo);
         }
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            uint32_t k = round_m(p, j, q, LR);
-            uint64_t x = v[j];
-            if (last) {
-                uint32_t e = bitrev32((m << L2) | k, log_n);
-                x = gl_mul(gl_mul(x, n_inv), tw_pow(tb.shl, tb.shh, e));
-            }
-            s[lds_addr(k)] = x;
-        }
+        for (int j = 0; j < R; j++) s[lds_addr(round_m(p, j, q, LR))] = v[j];
     }
 }
 
@@ -377,24 +458,27 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(t * G + gg, 0, q);
         if (p.o_lo) {
+            uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                uint32_t y = (p.o_lo * bitrev32(j, LR)) << (14 - q - LR);
-                v[gg][j] = gl_mul(v[gg][j], tb.t14f[y]);
-            }
+            for (int j = 1; j < R; j++) w[j] = tb.t14f[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            twiddle_rest<R>(v[gg], w);
         }
         dft_dit<R, false>(v[gg]);
+        if (last) {  // position P of block m times w_NExt^(e1 * P)
+            uint64_t lo[R], hi[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            uint32_t P = round_m(p, j, q, LR);
-            uint64_t x = v[gg][j];
-            if (last) {
-                uint32_t e = e1 * P;
-                if (e) x = gl_mul(x, tw_pow(tb.twl_f, tb.twh_f, e << shift_e));
-                dst[P] = x;
-            } else {
-                s[lds_addr(P)] = x;
+            for (int j = 0; j < R; j++) {
+                uint32_t e = (e1 * round_m(p, j, q, LR)) << shift_e;
+                lo[j] = tb.twl_f[e & 4095];
+                hi[j] = tb.twh_f[e >> 12];
             }
+            gl_mul_v<R>(lo, lo, hi);
+            gl_mul_v<R>(v[gg], v[gg], lo);
+#pragma unroll
+            for (int j = 0; j < R; j++) dst[round_m(p, j, q, LR)] = v[gg][j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; j++) s[lds_addr(round_m(p, j, q, LR))] = v[gg][j];
         }
     }
 }
