@@ -117,13 +117,30 @@ void pil2hip_batch_release(void);
 /* MerkleTreeGL::getGroupProof (merkleTreeGL.cpp:145-175) for nQueries row
  * indices d_idx (device memory) at once: proof q occupies
  * d_out[q*pw .. (q+1)*pw), pw = pil2hip_proof_words(nRows, nCols, arity, llv):
- * the nCols values of row idx (element (r, c) at d_src[c*srcColStride + r], the
- * column-major LDE layout), then per tree level above the last-level cut
- * (genMerkleProof, merkleTreeGL.cpp:157-175) the arity-1 sibling digests in
- * ascending order. An index >= nRows yields an all-zero proof. */
-int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t srcColStride, uint64_t nCols,
-                             const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+ * the nCols values of row idx (element (r, c) at d_src[r*rowStride + c*colStride]:
+ * rowStride 1, colStride NExt for the column-major LDE; rowStride = width,
+ * colStride 1 for a row-major source such as a FRI layer), then per tree level
+ * above the last-level cut (genMerkleProof, merkleTreeGL.cpp:157-175) the
+ * arity-1 sibling digests in ascending order. An index >= nRows yields an
+ * all-zero proof. FRI::proveFRIQueries (fri.hpp:131-139) is this call on a
+ * layer tree with indices reduced mod 2^currentBits. */
+int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t rowStride, uint64_t colStride,
+                             uint64_t nCols, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
                              const uint64_t *d_idx, uint64_t nQueries, void *stream);
+
+/* Merkle proof levels (MerkleTreeGL::getMerkleProofLength, merkleTreeGL.cpp:50-55)
+ * and words per group proof (nCols + (arity-1)*4*levels). */
+uint64_t pil2hip_proof_levels(uint64_t nRows, uint64_t arity, uint64_t llv);
+uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uint64_t llv);
+
+/* Proof-of-work grinding: Poseidon2Goldilocks<8>::grinding
+ * (poseidon2_goldilocks.cpp:177-232; gen_proof's runGrinding through
+ * starks_api_internal.hpp:21). *nonce = the smallest i with
+ * permute([challenge[0..3), i, 0, 0, 0, 0])[0] < 2^(64 - nBits), comparing the
+ * canonical value as the reference's tests do; challenge is 3 words in host
+ * memory. Synchronous on `stream`. -37 if no nonce exists below the reference's
+ * bound ceil(-128 ln 2 / ln(1 - 2^-nBits)). */
+int64_t pil2hip_grinding(uint64_t *nonce, const uint64_t *challenge, uint32_t nBits, void *stream);
 
 /* ---------------------------------------------------------------- serving --
  * A commit server: jobs arrive one by one (pil2hip_serve_submit), are admitted
@@ -194,10 +211,6 @@ uint64_t pil2hip_serve_job_bytes(uint64_t n_bits, uint64_t n_bits_ext, uint64_t 
 /* Drains admitted jobs, cancels queued ones, frees everything. */
 void pil2hip_serve_destroy(void *srv);
 
-/* Merkle proof levels (MerkleTreeGL::getMerkleProofLength, merkleTreeGL.cpp:50-55)
- * and words per group proof (nCols + (arity-1)*4*levels). */
-uint64_t pil2hip_proof_levels(uint64_t nRows, uint64_t arity, uint64_t llv);
-uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uint64_t llv);
 
 /* MerkleTreeGL::getRoot (merkleTreeGL.cpp:103-107): synchronous copy of the root
  * (4 words) into host memory. */

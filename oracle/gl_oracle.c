@@ -290,6 +290,21 @@ void oracle_p2_linear_hash(uint64_t *out4, const uint64_t *in, uint64_t size, ui
     for (int i = 0; i < 4; i++) out4[i] = o_canon(st[i]);
 }
 
+/* poseidon2_goldilocks.cpp:177-232 (grinding, width 8): the smallest nonce i with
+ * canonical(permute([c0, c1, c2, i, 0, 0, 0, 0])[0]) < 2^(64 - n_bits), tried
+ * in increasing order below `limit`; UINT64_MAX if none. The reference's tests
+ * (test_poseidon2_cpu.cpp:318-340, 448-465) state the contract this way. */
+uint64_t oracle_p2_grinding(const uint64_t *ch, uint32_t n_bits, uint64_t limit)
+{
+    uint64_t level = (uint64_t)1 << (64 - n_bits);
+    for (uint64_t i = 0; i < limit; i++) {
+        uint64_t st[8] = {ch[0], ch[1], ch[2], i, 0, 0, 0, 0};
+        oracle_p2_permute(st, 8);
+        if (o_canon(st[0]) < level) return i;
+    }
+    return UINT64_MAX;
+}
+
 /* goldilocks_tooling.hpp:9-21 */
 uint64_t oracle_tree_num_elements(uint64_t rows, uint64_t arity)
 {

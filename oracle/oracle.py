@@ -77,6 +77,8 @@ class Oracle:
         L.oracle_p2_linear_hash.argtypes = [_u64p, _u64p, _u64, _u64, _u32]
         L.oracle_tree_num_elements.restype = _u64
         L.oracle_tree_num_elements.argtypes = [_u64, _u64]
+        L.oracle_p2_grinding.restype = _u64
+        L.oracle_p2_grinding.argtypes = [_u64p, _u32, _u64]
         L.oracle_p2_merkletree.argtypes = [_u64p, _u64p, _u64, _u64, _u64, _u64, _u64]
         L.oracle_merkle_open.restype = _u64
         L.oracle_merkle_open.argtypes = [_u64p, _u64p, _u64p, _u64, _u64, _u64, _u64, _u64, _u64, _u64]
@@ -142,6 +144,10 @@ class Oracle:
         out = np.zeros(4, dtype=np.uint64)
         self.L.oracle_p2_linear_hash(_ptr(out), _ptr(v) if v.size else _ptr(np.zeros(1, np.uint64)), v.size, 1, width)
         return out
+
+    def grinding(self, challenge, n_bits, limit=1 << 24):
+        ch = np.ascontiguousarray(np.asarray(challenge, dtype=np.uint64)[:3])
+        return int(self.L.oracle_p2_grinding(_ptr(ch), n_bits, limit))
 
     def tree_num_elements(self, rows, arity):
         return self.L.oracle_tree_num_elements(rows, arity)
@@ -212,6 +218,8 @@ class RefLib:
         L.ref_linear_hash.restype = ctypes.c_int
         L.ref_linear_hash.argtypes = [_u64p, _u64p, _u64, _u32]
         L.ref_c3_mul.argtypes = [_u64p, _u64p, _u64p]
+        L.ref_grinding.restype = _u64
+        L.ref_grinding.argtypes = [_u64p, _u32]
         L.ref_num_threads.restype = ctypes.c_int
 
     @staticmethod
@@ -257,6 +265,10 @@ class RefLib:
         r = np.zeros(3, dtype=np.uint64)
         self.L.ref_c3_mul(_ptr(r), _ptr(a), _ptr(b))
         return r
+
+    def grinding(self, challenge, n_bits):
+        ch = np.ascontiguousarray(np.asarray(challenge, dtype=np.uint64)[:3])
+        return int(self.L.ref_grinding(_ptr(ch), n_bits))
 
     def num_threads(self):
         return self.L.ref_num_threads()

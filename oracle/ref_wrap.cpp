@@ -11,6 +11,7 @@
 //                     MerkleTreeGL::merkelize does)  merkleTreeGL.cpp:316-336
 //   ref_permute    -> Poseidon2Goldilocks<W>::permute poseidon2_goldilocks.hpp:252-272
 //   ref_linear_hash-> Poseidon2Goldilocks<W>::linearHash
+//   ref_grinding   -> Poseidon2GoldilocksGrinding::grinding (width 8)
 //   ref_c3_mul     -> Goldilocks3::mul               goldilocks_cubic_extension.hpp:232-245
 #include "goldilocks_base_field.hpp"
 #include "goldilocks_cubic_extension.hpp"
@@ -69,6 +70,18 @@ int ref_linear_hash(uint64_t *out4, const uint64_t *in, uint64_t size, uint32_t 
     }
     for (int k = 0; k < 4; k++) out4[k] = Goldilocks::toU64(o[k]);
     return 0;
+}
+
+// Poseidon2Goldilocks<8>::grinding (poseidon2_goldilocks.cpp:177-232); UINT64_MAX on failure
+uint64_t ref_grinding(const uint64_t *ch, uint32_t n_bits)
+{
+    uint64_t nonce = UINT64_MAX;
+    try {
+        Poseidon2GoldilocksGrinding::grinding(nonce, ch, n_bits);
+    } catch (...) {
+        return UINT64_MAX;
+    }
+    return nonce;
 }
 
 void ref_c3_mul(uint64_t *r, const uint64_t *a, const uint64_t *b)

@@ -605,14 +605,14 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
     // 0. row-major trace -> column-major scratch
     dim3 tg((unsigned)((N + TR_T - 1) / TR_T), (unsigned)((ncols + TR_T - 1) / TR_T));
     hipLaunchKernelGGL(k_transpose_rm2cm, tg, dim3(256), 0, st, trace, scratch, N, ncols, ncols, N);
-    if (ev) hipEventRecord(ev[0], st);
+    if (ev) (void)hipEventRecord(ev[0], st);
 
     if (log_n < 4) {  // columns shorter than 16 rows
         uint64_t ne = 1ull << log_ext;
         hipLaunchKernelGGL(k_lde_tiny, dim3((unsigned)((ne + 255) / 256), (unsigned)ncols), dim3(256), 0, st, scratch,
                            N, lde, ne, log_n, log_ext, n_inv, gl_inv(kW[log_n]), kW[log_ext]);
         for (int i = 1; i < 4; i++)
-            if (ev) hipEventRecord(ev[i], st);
+            if (ev) (void)hipEventRecord(ev[i], st);
         return (int)hipGetLastError();
     }
     // plan: n2 * B <= 2^14 values per block, n1 = N / n2 <= 2^10
@@ -630,7 +630,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
                            L2, lt, tb);
     }
-    if (ev) hipEventRecord(ev[1], st);
+    if (ev) (void)hipEventRecord(ev[1], st);
     // 2. fused block: DIF, coset scale, spread, DIT, twiddle
     {
         uint32_t log_tile = L2 + log_b;
@@ -638,7 +638,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         hipLaunchKernelGGL(k_lde_block, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
                            N << log_b, L1, L2, log_b, n_inv, tb);
     }
-    if (ev) hipEventRecord(ev[2], st);
+    if (ev) (void)hipEventRecord(ev[2], st);
     // 3. strided forward DIT over n1
     if (L1 > 0) {
         uint32_t l2b = L2 + log_b;
@@ -649,7 +649,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         hipLaunchKernelGGL(k_ntt_strided_fwd, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
                            N << log_b, L1, l2b, lt, tb);
     }
-    if (ev) hipEventRecord(ev[3], st);
+    if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }
 

@@ -4,12 +4,18 @@
 #include <stdint.h>
 
 #define P2_THREADS 256
+// tree levels with at most this many parents use 16 lanes per permutation
+#ifndef P2_COOP_MAX_PARENTS
+#define P2_COOP_MAX_PARENTS 16384
+#endif
 
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st);
 extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st);
-extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t ld, uint64_t ncols, const uint64_t *tree,
+extern "C" int p2_launch_grind(const uint64_t *ch, uint64_t level, uint64_t base, uint64_t count,
+                               uint64_t *found, hipStream_t st);
+extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols, const uint64_t *tree,
                                       uint64_t nrows, uint64_t arity, uint64_t cut, const uint64_t *idx,
                                       uint64_t nq, uint64_t pw, uint64_t *out, hipStream_t st);

@@ -273,6 +273,137 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_nodes(uint64_t *__restrict__ 
     for (int k = 0; k < 4; k++) o[k] = gl_canon(s[k]);
 }
 
+// ------------------------------------------------- cooperative levels ----
+// Upper tree levels have too few parents to fill the GPU with one permutation
+// per lane, so their time is the latency of one permutation. Here the 16 lanes
+// of a DPP row share one permutation: lane i holds state word i and lanes >= W
+// hold zero. Full rounds run all S-boxes at once; M4 reads the four words of
+// the lane's quad through DPP quad broadcasts; column sums and the state sum of
+// the partial rounds are DPP row rotations.
+
+// DPP reads of a VGPR need two wait states after its VALU write; values made by
+// inline asm pass through this barrier-nop first (the "+v" ties the order).
+__device__ __forceinline__ void dpp_ready(uint32_t &a, uint32_t &b, uint32_t &c)
+{
+    asm volatile("s_nop 1" : "+v"(a), "+v"(b), "+v"(c));
+}
+
+template <int CTRL>
+__device__ __forceinline__ lz dpp_lz(lz v)
+{
+    lz r;
+    r.l0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.l0, CTRL, 0xF, 0xF, false);
+    r.l1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.l1, CTRL, 0xF, 0xF, false);
+    r.c = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.c, CTRL, 0xF, 0xF, false);
+    return r;
+}
+
+constexpr int dpp_quad_bcast(int k) { return k | (k << 2) | (k << 4) | (k << 6); }
+constexpr int dpp_row_ror(int n) { return 0x120 + n; }
+
+__device__ __forceinline__ lz lz_add1(lz a, lz b)
+{
+    lz r;
+    lz_add_n<1>(&r, &a, &b);
+    return r;
+}
+
+__device__ __forceinline__ lz lz_sel(bool c, lz a, lz b) { return c ? a : b; }
+
+// M_E on the row: lane i gets (M4 x_quad)[i mod 4] + its column sum.
+template <int W>
+__device__ __forceinline__ lz ext_coop(uint64_t x, uint32_t i, bool live)
+{
+    lz v = lz_of(x);
+    dpp_ready(v.l0, v.l1, v.c);
+    lz x0 = dpp_lz<dpp_quad_bcast(0)>(v), x1 = dpp_lz<dpp_quad_bcast(1)>(v);
+    lz x2 = dpp_lz<dpp_quad_bcast(2)>(v), x3 = dpp_lz<dpp_quad_bcast(3)>(v);
+    lz t0 = lz_add1(x0, x1), t1 = lz_add1(x2, x3);
+    lz t2 = lz_add1(lz_shl<1>(x1), t1), t3 = lz_add1(lz_shl<1>(x3), t0);
+    lz t4 = lz_add1(lz_shl<2>(t1), t3), t5 = lz_add1(lz_shl<2>(t0), t2);
+    uint32_t q = i & 3;
+    lz y = q == 0 ? lz_add1(t3, t5) : q == 1 ? t5 : q == 2 ? lz_add1(t2, t4) : t4;
+    dpp_ready(y.l0, y.l1, y.c);
+    lz c = lz_add1(y, dpp_lz<dpp_row_ror(4)>(y));
+    dpp_ready(c.l0, c.l1, c.c);
+    c = lz_add1(c, dpp_lz<dpp_row_ror(8)>(c));
+    return lz_sel(live, lz_add1(y, c), lz_of(0));
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t p2_permute_coop(uint64_t s, uint32_t i)
+{
+    const bool live = i < (uint32_t)W;
+    const uint32_t li = live ? i : 0;
+    lz t = ext_coop<W>(s, i, live);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+        lz rc = lz_of(live ? P2Tab<W>::rc(r * W + li) : 0);
+        t = ext_coop<W>(pow7(lz_fold(lz_add1(t, rc))), i, live);
+    }
+    s = lz_fold(t);
+    const uint64_t d = live ? P2Tab<W>::d(li) : 0;
+#pragma unroll 1
+    for (int r = 0; r < 22; r++) {
+        uint64_t x0 = pow7(lz_fold(lz_add1(lz_of(s), lz_of(P2Tab<W>::rc(4 * W + r)))));
+        s = i == 0 ? x0 : s;
+        lz v = lz_of(s);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dpp_ready(v.l0, v.l1, v.c);
+            switch (k) {
+            case 0: v = lz_add1(v, dpp_lz<dpp_row_ror(1)>(v)); break;
+            case 1: v = lz_add1(v, dpp_lz<dpp_row_ror(2)>(v)); break;
+            case 2: v = lz_add1(v, dpp_lz<dpp_row_ror(4)>(v)); break;
+            default: v = lz_add1(v, dpp_lz<dpp_row_ror(8)>(v)); break;
+            }
+        }
+        uint64_t sum = lz_fold(v);
+        s = live ? p2_mul_add(s, d, (uint32_t)sum, (uint32_t)(sum >> 32)) : 0;
+    }
+    t = lz_of(s);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+        lz rc = lz_of(live ? P2Tab<W>::rc(4 * W + 22 + r * W + li) : 0);
+        t = ext_coop<W>(pow7(lz_fold(lz_add1(t, rc))), i, live);
+    }
+    return lz_fold(t);
+}
+
+// One tree level, 16 lanes per parent (same layout and pad rule as k_p2_nodes).
+template <int W>
+__global__ __launch_bounds__(P2_THREADS) void k_p2_nodes_coop(uint64_t *__restrict__ tree, uint64_t in_off,
+                                                              uint64_t pending, uint64_t pad, uint64_t nnext)
+{
+    constexpr int ARITY = W / 4;
+    uint64_t par = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    uint32_t i = threadIdx.x & 15;
+    bool active = par < nnext;  // whole rows are active or not
+    uint64_t s = 0;
+    if (active && i < (uint32_t)W && par * ARITY + (i >> 2) < pending) s = tree[in_off + par * W + i];
+    if (active && par == nnext - 1 && i == 0) {
+        for (uint64_t z = 0; z < pad * 4; z++) tree[in_off + pending * 4 + z] = 0;
+    }
+    s = p2_permute_coop<W>(s, i);
+    if (active && i < 4) tree[in_off + (pending + pad) * 4 + par * 4 + i] = gl_canon(s);
+}
+
+// ----------------------------------------------------------- grinding ----
+// Poseidon2Goldilocks<8>::grinding (poseidon2_goldilocks.cpp:177-232): nonce i
+// is valid when permute([c0, c1, c2, i, 0, 0, 0, 0])[0] < 2^(64 - n_bits). One
+// lane per nonce of the window [base, base + count); valid nonces go through an
+// atomic minimum, so a window reports its smallest valid nonce.
+__global__ __launch_bounds__(P2_THREADS) void k_p2_grind(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t level,
+                                                         uint64_t base, uint64_t count,
+                                                         unsigned long long *__restrict__ found)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint64_t s[8] = {c0, c1, c2, base + k, 0, 0, 0, 0};
+    p2_permute<8>(s);
+    if (gl_canon(s[0]) < level) atomicMin(found, (unsigned long long)(base + k));
+}
+
 // Single permutation over `count` states stored consecutively (W words each).
 template <int W>
 __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
@@ -289,11 +420,12 @@ __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
 
 // ------------------------------------------------------------ openings ----
 // Group proofs (MerkleTreeGL::getGroupProof, merkleTreeGL.cpp:145-175) for many
-// queries: block q writes proof q at out + q*pw: the nCols values of LDE row
-// idx (column-major source, element (r, c) at src[c*ld + r]), then for every
+// queries: block q writes proof q at out + q*pw: the nCols values of source row
+// idx (element (r, c) at src[r*rs + c*cs]), then for every
 // level above the last-level-verification cut the arity-1 sibling digests in
 // ascending child order. A query index >= nrows yields an all-zero proof.
-__global__ __launch_bounds__(256) void k_group_proofs(const uint64_t *__restrict__ src, uint64_t ld, uint64_t ncols,
+__global__ __launch_bounds__(256) void k_group_proofs(const uint64_t *__restrict__ src, uint64_t rs, uint64_t cs,
+                                                      uint64_t ncols,
                                                       const uint64_t *__restrict__ tree, uint64_t nrows,
                                                       uint64_t arity, uint64_t cut, const uint64_t *__restrict__ idx,
                                                       uint64_t pw, uint64_t *__restrict__ out)
@@ -305,7 +437,7 @@ __global__ __launch_bounds__(256) void k_group_proofs(const uint64_t *__restrict
         for (uint64_t k = threadIdx.x; k < pw; k += blockDim.x) o[k] = 0;
         return;
     }
-    for (uint64_t c = threadIdx.x; c < ncols; c += blockDim.x) o[c] = gl_canon(src[c * ld + r]);
+    for (uint64_t c = threadIdx.x; c < ncols; c += blockDim.x) o[c] = gl_canon(src[r * rs + c * cs]);
     // siblings: level by level; thread t copies word t of each level's (arity-1)*4 words
     uint64_t per = (arity - 1) * 4;
     uint64_t t = threadIdx.x;
@@ -331,13 +463,23 @@ __global__ void k_copy_root(const uint64_t *__restrict__ src, uint64_t *__restri
 }
 
 // ------------------------------------------------------------ launchers ----
-extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t ld, uint64_t ncols, const uint64_t *tree,
+extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols, const uint64_t *tree,
                                       uint64_t nrows, uint64_t arity, uint64_t cut, const uint64_t *idx,
                                       uint64_t nq, uint64_t pw, uint64_t *out, hipStream_t st)
 {
     if (nq == 0) return 0;
-    hipLaunchKernelGGL(k_group_proofs, dim3((unsigned)nq), dim3(256), 0, st, src, ld, ncols, tree, nrows, arity, cut,
+    hipLaunchKernelGGL(k_group_proofs, dim3((unsigned)nq), dim3(256), 0, st, src, rs, cs, ncols, tree, nrows, arity, cut,
                        idx, pw, out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int p2_launch_grind(const uint64_t *ch, uint64_t level, uint64_t base, uint64_t count,
+                               uint64_t *found, hipStream_t st)
+{
+    if (count == 0) return 0;
+    dim3 g((unsigned)((count + P2_THREADS - 1) / P2_THREADS));
+    hipLaunchKernelGGL(k_p2_grind, g, dim3(P2_THREADS), 0, st, ch[0], ch[1], ch[2], level, base, count,
+                       (unsigned long long *)found);
     return (int)hipGetLastError();
 }
 
@@ -368,12 +510,22 @@ extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nl
     while (pending > 1) {
         uint64_t pad = (arity - pending % arity) % arity;
         uint64_t next = (pending + arity - 1) / arity;
-        dim3 g((unsigned)((next + P2_THREADS - 1) / P2_THREADS));
-        switch (width) {
-        case 8: hipLaunchKernelGGL(k_p2_nodes<8>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
-        case 12: hipLaunchKernelGGL(k_p2_nodes<12>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
-        case 16: hipLaunchKernelGGL(k_p2_nodes<16>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
-        default: return -1;
+        if (next <= P2_COOP_MAX_PARENTS) {  // latency-bound level: 16 lanes per parent
+            dim3 g((unsigned)((next * 16 + P2_THREADS - 1) / P2_THREADS));
+            switch (width) {
+            case 8: hipLaunchKernelGGL(k_p2_nodes_coop<8>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            case 12: hipLaunchKernelGGL(k_p2_nodes_coop<12>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            case 16: hipLaunchKernelGGL(k_p2_nodes_coop<16>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            default: return -1;
+            }
+        } else {
+            dim3 g((unsigned)((next + P2_THREADS - 1) / P2_THREADS));
+            switch (width) {
+            case 8: hipLaunchKernelGGL(k_p2_nodes<8>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            case 12: hipLaunchKernelGGL(k_p2_nodes<12>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            case 16: hipLaunchKernelGGL(k_p2_nodes<16>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;
+            default: return -1;
+            }
         }
         off += (pending + pad) * 4;
         pending = next;

@@ -235,7 +235,7 @@ struct Server {
         int64_t rc = enqueue(j, sl, streams[best]);
         if (rc) {  // launch-time failure: the job completes with the error
             stream_load[best] -= j.work;
-            hipStreamSynchronize(streams[best]);
+            (void)hipStreamSynchronize(streams[best]);
             sl.busy = false;
             free_transient(j);
             free_resident(j);
@@ -283,8 +283,8 @@ struct Server {
             memcpy(h_q, j.queries.data(), 8 * nq);
             if (hipMemcpyAsync(d_idx, h_q, 8 * nq, hipMemcpyHostToDevice, st) != hipSuccess)
                 return api_fail(kErrArena, "query copy failed");
-            rc = pil2hip_group_proofs(d_proofs, j.d_lde, ne, r.n_cols, j.d_nodes, ne, cfg.arity, cfg.llv, d_idx, nq,
-                                      st);
+            rc = pil2hip_group_proofs(d_proofs, j.d_lde, 1, ne, r.n_cols, j.d_nodes, ne, cfg.arity, cfg.llv, d_idx,
+                                      nq, st);
             if (rc) return rc;
             if (hipMemcpyAsync(h_pr, d_proofs, 8 * nq * j.L.pw, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return api_fail(kErrArena, "proof copy failed");
@@ -320,7 +320,7 @@ struct Server {
 
     void loop()
     {
-        hipSetDevice(device);
+        (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             bool progressed = false;
@@ -418,10 +418,24 @@ void *pil2hip_serve_create(const Pil2ServeConfig *cfg)
     s->arena.init(base, cfg->hbm_budget_bytes);
     s->streams.resize(cfg->n_streams);
     s->stream_load.assign(cfg->n_streams, 0.0);
-    for (auto &st : s->streams) hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    hipStreamCreateWithFlags(&s->open_stream, hipStreamNonBlocking);
+    hipError_t ce = hipSuccess;
+    for (auto &st : s->streams)
+        if (ce == hipSuccess) ce = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (ce == hipSuccess) ce = hipStreamCreateWithFlags(&s->open_stream, hipStreamNonBlocking);
     s->slots.resize(cfg->max_inflight);
-    for (auto &sl : s->slots) hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    for (auto &sl : s->slots)
+        if (ce == hipSuccess) ce = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    if (ce != hipSuccess) {  // no thread yet: release what exists and report
+        for (auto st : s->streams)
+            if (st) (void)hipStreamDestroy(st);
+        if (s->open_stream) (void)hipStreamDestroy(s->open_stream);
+        for (auto &sl : s->slots)
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (base) (void)hipFree(base);
+        delete s;
+        api_fail(-100 - (int64_t)ce, std::string("server streams/events: ") + hipGetErrorString(ce));
+        return nullptr;
+    }
     s->thread = std::thread([s] { s->loop(); });
     return s;
 }
@@ -519,7 +533,8 @@ int64_t pil2hip_serve_open(void *srv, uint64_t job_id, const uint64_t *h_queries
     int64_t rc = 0;
     if (hipMemcpyAsync(d_idx, h_queries, 8 * n_queries, hipMemcpyHostToDevice, st) != hipSuccess)
         rc = api_fail(kErrArena, "query copy failed");
-    if (!rc) rc = pil2hip_group_proofs(d_out, d_lde, ne, nc, d_nodes, ne, s->cfg.arity, s->cfg.llv, d_idx, n_queries, st);
+    if (!rc)
+        rc = pil2hip_group_proofs(d_out, d_lde, 1, ne, nc, d_nodes, ne, s->cfg.arity, s->cfg.llv, d_idx, n_queries, st);
     if (!rc && hipMemcpyAsync(h_proofs, d_out, 8 * n_queries * pw, hipMemcpyDeviceToHost, st) != hipSuccess)
         rc = api_fail(kErrArena, "proof copy failed");
     hipError_t e = hipStreamSynchronize(st);
@@ -581,16 +596,16 @@ void pil2hip_serve_destroy(void *srv)
     }
     if (s->thread.joinable()) s->thread.join();
     for (auto st : s->streams) {
-        hipStreamSynchronize(st);
-        hipStreamDestroy(st);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     }
-    hipStreamDestroy(s->open_stream);
+    (void)hipStreamDestroy(s->open_stream);
     for (auto &sl : s->slots) {
-        hipEventDestroy(sl.ev);
-        if (sl.pinned) hipHostFree(sl.pinned);
+        (void)hipEventDestroy(sl.ev);
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
     }
-    for (auto p : s->old_pinned) hipHostFree(p);
-    if (s->arena.base) hipFree(s->arena.base);
+    for (auto p : s->old_pinned) (void)hipHostFree(p);
+    if (s->arena.base) (void)hipFree(s->arena.base);
     delete s;
 }
 

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -168,17 +169,17 @@ static int64_t commit_impl(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d
     if ((rc = check_arity(arity))) return rc;
     if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
     if (!d_nodes) return fail(-13, "null buffer");
-    if (ev) hipEventRecord(g_ev[0], st);
+    if (ev) (void)hipEventRecord(g_ev[0], st);
     if ((rc = lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st, ev))) return rc;
     uint64_t NExt = 1ull << nBitsExt;
     uint32_t width = (uint32_t)(4 * arity);
     int e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
     if (e) return fail(-100 - e, "leaf hashing launch failed");
-    if (ev) hipEventRecord(g_ev[5], st);
+    if (ev) (void)hipEventRecord(g_ev[5], st);
     e = p2_launch_tree_levels(width, d_nodes, NExt, st);
     if (e) return fail(-100 - e, "tree level launch failed");
     if (ev) {
-        hipEventRecord(g_ev[6], st);
+        HIP_TRY(hipEventRecord(g_ev[6], st), "timing event");
         HIP_TRY(hipEventSynchronize(g_ev[6]), "timing sync");
         for (int i = 0; i < kPhases; i++) {
             float ms = 0;
@@ -246,8 +247,8 @@ void free_device_buffers(void *d_buffers)
 {
     DeviceBuffersHip *b = (DeviceBuffersHip *)d_buffers;
     if (!b) return;
-    hipStreamSynchronize(b->stream);
-    hipStreamDestroy(b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -350,8 +351,10 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, 
     HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
     t_capturing = true;
     int64_t rc = 0;
-    hipEventRecord(g_batch_events[kMaxBatchStreams], cap);
-    for (uint32_t s = 0; s < S; s++) hipStreamWaitEvent(g_batch_streams[s], g_batch_events[kMaxBatchStreams], 0);
+    hipError_t fe = hipEventRecord(g_batch_events[kMaxBatchStreams], cap);
+    for (uint32_t s = 0; s < S && fe == hipSuccess; s++)
+        fe = hipStreamWaitEvent(g_batch_streams[s], g_batch_events[kMaxBatchStreams], 0);
+    if (fe != hipSuccess) rc = fail(-100 - (int64_t)fe, std::string("graph fork: ") + hipGetErrorString(fe));
     for (uint64_t k = 0; k < n && !rc; k++) {
         const Pil2CommitJob &j = jobs[order[k]];
         hipStream_t st = g_batch_streams[assign[order[k]]];
@@ -363,20 +366,21 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, 
             if (e) rc = fail(-100 - e, "root copy launch failed");
         }
     }
-    for (uint32_t s = 0; s < S; s++) {
-        hipEventRecord(g_batch_events[s], g_batch_streams[s]);
-        hipStreamWaitEvent(cap, g_batch_events[s], 0);
+    for (uint32_t s = 0; s < S; s++) {  // join every branch (also after an error, to end the capture cleanly)
+        hipError_t je = hipEventRecord(g_batch_events[s], g_batch_streams[s]);
+        if (je == hipSuccess) je = hipStreamWaitEvent(cap, g_batch_events[s], 0);
+        if (je != hipSuccess && !rc) rc = fail(-100 - (int64_t)je, std::string("graph join: ") + hipGetErrorString(je));
     }
     t_capturing = false;
     hipGraph_t graph = nullptr;
     hipError_t ee = hipStreamEndCapture(cap, &graph);
     if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("end capture: ") + hipGetErrorString(ee));
     ee = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("graph instantiate: ") + hipGetErrorString(ee));
     return 0;
 }
@@ -428,30 +432,35 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
         if ((rc = batch_capture(jobs, n_jobs, S, d_roots, &ex))) break;
         float ms = 0;
         if (cands.size() > 1) {  // autotune: one warm launch, one timed launch
-            hipGraphLaunch(ex, st);
-            hipEventRecord(e0, st);
-            hipGraphLaunch(ex, st);
-            hipEventRecord(e1, st);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&ms, e0, e1);
+            hipError_t te = hipGraphLaunch(ex, st);
+            if (te == hipSuccess) te = hipEventRecord(e0, st);
+            if (te == hipSuccess) te = hipGraphLaunch(ex, st);
+            if (te == hipSuccess) te = hipEventRecord(e1, st);
+            if (te == hipSuccess) te = hipEventSynchronize(e1);
+            if (te == hipSuccess) te = hipEventElapsedTime(&ms, e0, e1);
+            if (te != hipSuccess) {
+                (void)hipGraphExecDestroy(ex);
+                rc = fail(-100 - (int64_t)te, std::string("autotune: ") + hipGetErrorString(te));
+                break;
+            }
         }
         if (ms < best_ms) {
-            if (best) hipGraphExecDestroy(best);
+            if (best) (void)hipGraphExecDestroy(best);
             best = ex;
             best_ms = ms;
             best_s = S;
         } else {
-            hipGraphExecDestroy(ex);
+            (void)hipGraphExecDestroy(ex);
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) {
-        if (best) hipGraphExecDestroy(best);
+        if (best) (void)hipGraphExecDestroy(best);
         return rc;
     }
     if (g_batch_graphs.size() >= 8) {  // small cache: drop the oldest graph
-        hipGraphExecDestroy(g_batch_graphs.front().exec);
+        (void)hipGraphExecDestroy(g_batch_graphs.front().exec);
         g_batch_graphs.erase(g_batch_graphs.begin());
     }
     g_batch_graphs.push_back({key, best_s, best});
@@ -462,7 +471,7 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
 void pil2hip_batch_release(void)
 {
     std::lock_guard<std::mutex> lk(g_batch_mu);
-    for (auto &g : g_batch_graphs) hipGraphExecDestroy(g.exec);
+    for (auto &g : g_batch_graphs) (void)hipGraphExecDestroy(g.exec);
     g_batch_graphs.clear();
 }
 
@@ -484,8 +493,8 @@ uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uin
     return nCols + (arity - 1) * 4 * pil2hip_proof_levels(nRows, arity, llv);
 }
 
-int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t srcColStride, uint64_t nCols,
-                             const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t rowStride, uint64_t colStride,
+                             uint64_t nCols, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
                              const uint64_t *d_idx, uint64_t nQueries, void *stream)
 {
     int64_t rc = check_arity(arity);
@@ -496,10 +505,50 @@ int64_t pil2hip_group_proofs(uint64_t *d_out, const uint64_t *d_src, uint64_t sr
     uint64_t cut = 1;
     for (uint64_t k = 0; k < llv && cut <= nRows; k++) cut *= arity;
     uint64_t pw = pil2hip_proof_words(nRows, nCols, arity, llv);
-    int e = p2_launch_group_proofs(d_src, srcColStride, nCols, d_nodes, nRows, arity, cut, d_idx, nQueries, pw, d_out,
-                                   (hipStream_t)stream);
+    int e = p2_launch_group_proofs(d_src, rowStride, colStride, nCols, d_nodes, nRows, arity, cut, d_idx, nQueries, pw,
+                                   d_out, (hipStream_t)stream);
     if (e) return fail(-100 - e, "group proof launch failed");
     return 0;
+}
+
+// Grinding (Poseidon2Goldilocks<8>::grinding, poseidon2_goldilocks.cpp:177-232,
+// called by gen_proof through starks_api_internal.hpp:21): the smallest nonce
+// whose permuted state word 0 is below 2^(64 - nBits), searched in windows of
+// increasing nonces until one holds a valid nonce, up to the reference's bound
+// N = ceil(-128 ln 2 / ln(1 - 2^-nBits)).
+static uint64_t *g_grind_found[kMaxDevices];
+static std::mutex g_grind_mu;
+
+int64_t pil2hip_grinding(uint64_t *nonce, const uint64_t *challenge, uint32_t nBits, void *stream)
+{
+    if (!nonce || !challenge) return fail(-13, "null argument");
+    if (nBits < 1 || nBits > 63) return fail(-36, "grinding bits must be 1..63");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_grind_mu);
+    if (!g_grind_found[dev]) HIP_TRY(hipMalloc(&g_grind_found[dev], 8), "hipMalloc(grinding)");
+    uint64_t *d_found = g_grind_found[dev];
+    const double eps = std::ldexp(1.0, -(int)nBits);
+    const double total = -128.0 * std::log(2.0) / std::log1p(-eps);
+    const uint64_t limit = total >= 1.8e19 ? UINT64_MAX : (uint64_t)std::ceil(total);
+    const uint64_t level = 1ull << (64 - nBits);
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t none = UINT64_MAX, got = UINT64_MAX;
+    HIP_TRY(hipMemcpyAsync(d_found, &none, 8, hipMemcpyHostToDevice, st), "grinding reset");
+    const uint64_t window = 1ull << 22;
+    for (uint64_t base = 0; base < limit; base += window) {
+        uint64_t count = limit - base < window ? limit - base : window;
+        int e = p2_launch_grind(challenge, level, base, count, d_found, st);
+        if (e) return fail(-100 - e, "grinding launch failed");
+        HIP_TRY(hipMemcpyAsync(&got, d_found, 8, hipMemcpyDeviceToHost, st), "grinding result");
+        HIP_TRY(hipStreamSynchronize(st), "grinding sync");
+        if (got != UINT64_MAX) {
+            *nonce = got;
+            return 0;
+        }
+    }
+    return fail(-37, "grinding: no valid nonce below the bound");
 }
 
 void pil2hip_timing_enable(bool on) { g_timing = on; }
@@ -612,7 +661,7 @@ static int64_t write_custom_commit_impl(void *root, uint64_t arity, uint64_t nBi
     bool want_file = bufferFile && bufferFile[0];
     uint64_t *d_trace = nullptr, *d_scratch = nullptr, *d_lde = nullptr, *d_nodes = nullptr, *d_rm = nullptr;
     auto cleanup = [&]() {
-        hipFree(d_trace); hipFree(d_scratch); hipFree(d_lde); hipFree(d_nodes); hipFree(d_rm);
+        (void)hipFree(d_trace); (void)hipFree(d_scratch); (void)hipFree(d_lde); (void)hipFree(d_nodes); (void)hipFree(d_rm);
     };
     uint64_t tsz = N * nCols, lsz = NExt * nCols;
     if (hipMalloc(&d_trace, (tsz ? tsz : 1) * 8) != hipSuccess || hipMalloc(&d_scratch, (tsz ? tsz : 1) * 8) != hipSuccess ||

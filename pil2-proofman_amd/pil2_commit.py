@@ -50,8 +50,9 @@ _SIGS = {
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_batch_release": (None, []),
-    "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
+    "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
     "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
+    "pil2hip_grinding": (_i64, [_vp, _vp, ctypes.c_uint32, _vp]),
     "pil2hip_proof_words": (_u64, [_u64, _u64, _u64, _u64]),
     "pil2hip_serve_create": (_vp, [_vp]),
     "pil2hip_serve_submit": (_i64, [_vp, _vp]),
@@ -294,6 +295,17 @@ class FRI:
         return dst
 
     @staticmethod
+    def prove_queries(tree, queries, current_bits, stream=None):
+        """FRI::proveFRIQueries (fri.hpp:131-139): group proofs on a layer tree at
+        the query indices reduced mod 2^current_bits. Returns (n, proof_words) uint64."""
+        q = np.asarray(queries, dtype=np.uint64) % np.uint64(1 << current_bits)
+        pw = proof_words(tree.height, tree.width, tree.arity, tree.last_level_verification)
+        out = group_proofs(tree.source, tree.height, tree.width, tree.nodes, tree.arity,
+                           tree.last_level_verification, to_device(q, tree.source.device),
+                           row_stride=tree.row_stride, col_stride=tree.col_stride, stream=stream)
+        return to_host(out)[:q.size * pw].reshape(q.size, pw)
+
+    @staticmethod
     def merkelize(tree, pol, current_bits, next_bits, stream=None):
         """fri.hpp:102-109: regroup (getTransposed) into tree.source, merkelize, root."""
         degree = 1 << current_bits
@@ -391,12 +403,14 @@ def proof_words(n_rows, n_cols, arity, llv):
     return int(_lib.pil2hip_proof_words(n_rows, n_cols, arity, llv))
 
 
-def group_proofs(lde, n_rows, n_cols, nodes, arity, llv, idx, stream=None):
+def group_proofs(src, n_rows, n_cols, nodes, arity, llv, idx, row_stride=1, col_stride=None, stream=None):
     """Device getGroupProof for many row indices (idx: device int64 tensor).
-    Returns a device tensor of len(idx) * proof_words(...) words."""
+    The source defaults to the column-major LDE layout (row_stride 1, col_stride
+    n_rows). Returns a device tensor of len(idx) * proof_words(...) words."""
     pw = proof_words(n_rows, n_cols, arity, llv)
-    out = empty(max(idx.numel() * pw, 1), lde.device)
-    _check(_lib.pil2hip_group_proofs(_ptr(out), _ptr(lde), n_rows, n_cols, _ptr(nodes), n_rows, arity, llv,
+    cs = n_rows if col_stride is None else col_stride
+    out = empty(max(idx.numel() * pw, 1), src.device)
+    _check(_lib.pil2hip_group_proofs(_ptr(out), _ptr(src), row_stride, cs, n_cols, _ptr(nodes), n_rows, arity, llv,
                                      _ptr(idx), idx.numel(), _stream(stream)))
     return out
 
@@ -512,3 +526,12 @@ class ServeEngine:
 
 def serve_job_bytes(n_bits, n_bits_ext, n_cols, arity, n_queries, llv, device_trace=False):
     return int(_lib.pil2hip_serve_job_bytes(n_bits, n_bits_ext, n_cols, arity, n_queries, llv, device_trace))
+
+
+def grinding(challenge, n_bits, stream=None):
+    """Poseidon2Goldilocks<8>::grinding (poseidon2_goldilocks.cpp:177-232): the
+    smallest nonce whose permuted state word 0 is below 2^(64 - n_bits)."""
+    ch = np.ascontiguousarray(np.asarray(challenge, dtype=np.uint64)[:3])
+    out = np.zeros(1, dtype=np.uint64)
+    _check(_lib.pil2hip_grinding(out.ctypes.data_as(_vp), ch.ctypes.data_as(_vp), n_bits, _stream(stream)))
+    return int(out[0])

@@ -297,6 +297,12 @@ def test_fri_fold_all_layers(pc, oracle, nbe):
             expect = oracle.merkletree(aux.reshape(1 << nxt, w), 3)
             assert (root == expect[-4:]).all(), "root %d" % s
             assert (pc.to_host(tree.source) == aux).all()
+            # FRI::proveFRIQueries on this layer tree: indices reduced mod 2^current_bits
+            queries = [0, 5, (1 << cur) - 1, 123456789, 2**40 + 17]
+            proofs = pc.FRI.prove_queries(tree, queries, nxt)
+            for qi, q in enumerate(queries):
+                exp = oracle.merkle_open(expect, aux.reshape(1 << nxt, w), 3, 2, q % (1 << nxt))
+                assert (proofs[qi] == exp).all(), ("fri query", s, q)
 
 
 # ------------------------------------------------------------- openings --
@@ -358,3 +364,22 @@ def test_plain_large_commit_sampled(pc, oracle):
     lde2, nodes2 = pc.commit(d_trace, arity, nb, nbe, c, lde=lde, nodes=nodes)
     root2 = pc.MerkleTreeGL(arity, 0, True, ne, c, nodes=nodes2).get_root()
     assert (root1 == root2).all()
+
+
+# -------------------------------------------------------------- grinding --
+@pytest.mark.parametrize("ch,nb", [([0x1234567890ABCDEF, 0xFEDCBA0987654321, 0x0123456789ABCDEF], 8), ([1, 2, 3], 1),
+                                   ([0, 0, 0], 10), ([P - 1, 2**64 - 1, 12345], 12), ([5, 6, 7], 16)])
+def test_grinding_matches_oracle(pc, oracle, ch, nb):
+    """Smallest valid nonce (Poseidon2Goldilocks<8>::grinding) equals the oracle's
+    exhaustive search; the first two cases are the reference's own test inputs."""
+    assert pc.grinding(ch, nb) == oracle.grinding(ch, nb)
+
+
+def test_grinding_large_matches_compiled_reference(pc):
+    """A 20-bit search (about a million permutations, several windows of the
+    GPU search) against the reference's own grinding when it is built."""
+    from oracle.oracle import RefLib
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built")
+    ch = [0x0123456789ABCDEF, 42, 2**63 + 5]
+    assert pc.grinding(ch, 20) == RefLib().grinding(ch, 20)

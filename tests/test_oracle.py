@@ -263,3 +263,32 @@ def test_compiled_reference_agrees_with_oracle(oracle):
         assert (ref.merkletree(src, a, n) == oracle.merkletree(src, a)).all()
     x, y = datagen.uniform(3, 1), datagen.uniform(3, 2)
     assert (ref.c3_mul(x, y) == oracle.c3_mul(x, y)).all()
+
+
+# -------------------------------------------------------------- grinding --
+# (challenge, n_bits): the reference's own cases (test_poseidon2_cpu.cpp:318-340
+# and :448-465) and two more, one with non-canonical challenge words
+GRIND_CASES = [([0x1234567890ABCDEF, 0xFEDCBA0987654321, 0x0123456789ABCDEF], 8), ([1, 2, 3], 1),
+               ([0, 0, 0], 10), ([P - 1, 2**64 - 1, 12345], 12)]
+
+
+def test_grinding_contract_and_minimality(oracle):
+    """The nonce satisfies the grinding contract (state word 0 of the width-8
+    permutation of [challenge, nonce, 0...] below 2^(64-n)) and no smaller
+    nonce does (checked exhaustively through the pinned permutation)."""
+    for ch, nb in GRIND_CASES:
+        n = oracle.grinding(ch, nb)
+        assert n != 2**64 - 1
+        level = 2 ** (64 - nb)
+        assert int(oracle.permute([ch[0], ch[1], ch[2], n, 0, 0, 0, 0])[0]) < level
+        for i in range(n):
+            assert int(oracle.permute([ch[0], ch[1], ch[2], i, 0, 0, 0, 0])[0]) >= level, (ch, nb, i)
+
+
+def test_grinding_matches_compiled_reference(oracle):
+    from oracle.oracle import RefLib
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref = RefLib()
+    for ch, nb in GRIND_CASES:
+        assert ref.grinding(ch, nb) == oracle.grinding(ch, nb), (ch, nb)
