@@ -142,6 +142,22 @@ uint64_t pil2hip_proof_words(uint64_t nRows, uint64_t nCols, uint64_t arity, uin
  * bound ceil(-128 ln 2 / ln(1 - 2^-nBits)). */
 int64_t pil2hip_grinding(uint64_t *nonce, const uint64_t *challenge, uint32_t nBits, void *stream);
 
+/* Fiat-Shamir transcript (TranscriptGL, transcriptGL.hpp:11-58 /
+ * transcriptGL.cpp:4-119) with its state in device memory: Poseidon2 of width
+ * 4*arity, rate 4*(arity-1), capacity 4. put absorbs words from device memory
+ * (e.g. a Merkle root where the commit left it) or from host memory;
+ * get_field squeezes 3 words (a cubic challenge); get_state returns the first
+ * n state words; get_permutations draws n query indices of nBits each from
+ * ceil(n*nBits/63) squeezed fields, 63 low bits per field. Words returned are
+ * canonical (the reference returns its internal word, equal mod p). */
+void *pil2hip_transcript_create(uint64_t arity);
+int64_t pil2hip_transcript_put(void *t, const uint64_t *d_in, uint64_t n, void *stream);
+int64_t pil2hip_transcript_put_host(void *t, const uint64_t *h_in, uint64_t n, void *stream);
+int64_t pil2hip_transcript_get_field(void *t, uint64_t *h_out, void *stream);
+int64_t pil2hip_transcript_get_state(void *t, uint64_t *h_out, uint64_t n, void *stream);
+int64_t pil2hip_transcript_get_permutations(void *t, uint64_t *h_out, uint64_t n, uint64_t nBits, void *stream);
+void pil2hip_transcript_destroy(void *t);
+
 /* ---------------------------------------------------------------- serving --
  * A commit server: jobs arrive one by one (pil2hip_serve_submit), are admitted
  * first-come first-served as in-flight slots and HBM budget free up, run

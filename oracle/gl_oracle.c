@@ -305,6 +305,78 @@ uint64_t oracle_p2_grinding(const uint64_t *ch, uint32_t n_bits, uint64_t limit)
     return UINT64_MAX;
 }
 
+/* TranscriptGL (transcriptGL.hpp:11-58, transcriptGL.cpp:4-119): sponge of width
+ * W = 4*arity, rate W-4, capacity 4; outputs canonical. */
+typedef struct {
+    uint64_t state[16], pending[12], out[16];
+    uint32_t pc, oc, W;
+} OTranscript;
+
+void oracle_tr_init(OTranscript *t, uint32_t arity)
+{
+    memset(t, 0, sizeof(*t));
+    t->W = 4 * arity;
+}
+
+static void o_tr_update(OTranscript *t)   /* _updateState, transcriptGL.cpp:12-45 */
+{
+    uint32_t rate = t->W - 4;
+    uint64_t in[16];
+    for (uint32_t i = 0; i < rate; i++) in[i] = i < t->pc ? t->pending[i] : 0;
+    for (uint32_t i = 0; i < 4; i++) in[rate + i] = t->state[i];
+    oracle_p2_permute(in, t->W);
+    for (uint32_t i = 0; i < t->W; i++) t->out[i] = t->state[i] = o_canon(in[i]);
+    t->oc = t->W;
+    for (uint32_t i = 0; i < rate; i++) t->pending[i] = 0;
+    t->pc = 0;
+}
+
+void oracle_tr_put(OTranscript *t, const uint64_t *in, uint64_t n)   /* put / _add1, :4-10, :47-56 */
+{
+    for (uint64_t k = 0; k < n; k++) {
+        t->pending[t->pc++] = in[k];
+        t->oc = 0;
+        if (t->pc == t->W - 4) o_tr_update(t);
+    }
+}
+
+static uint64_t o_tr_field1(OTranscript *t)   /* getFields1, :81-90 */
+{
+    if (t->oc == 0) o_tr_update(t);
+    uint64_t r = t->out[(t->W - t->oc) % t->W];
+    t->oc--;
+    return r;
+}
+
+void oracle_tr_get_field(OTranscript *t, uint64_t *out3)   /* getField, :58-65 */
+{
+    for (int i = 0; i < 3; i++) out3[i] = o_tr_field1(t);
+}
+
+void oracle_tr_get_state(OTranscript *t, uint64_t *out, uint32_t n)   /* getState, :67-79 */
+{
+    if (t->pc > 0) o_tr_update(t);
+    for (uint32_t i = 0; i < n; i++) out[i] = t->state[i];
+}
+
+void oracle_tr_get_permutations(OTranscript *t, uint64_t *res, uint64_t n, uint64_t nbits)   /* :92-118 */
+{
+    uint64_t nfields = (n * nbits - 1) / 63 + 1, cur = 0, bit = 0;
+    uint64_t *f = (uint64_t *)malloc(nfields * 8);
+    for (uint64_t i = 0; i < nfields; i++) f[i] = o_tr_field1(t);
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t a = 0;
+        for (uint64_t j = 0; j < nbits; j++) {
+            if ((f[cur] >> bit) & 1) a += (uint64_t)1 << j;
+            if (++bit == 63) { bit = 0; cur++; }
+        }
+        res[i] = a;
+    }
+    free(f);
+}
+
+uint64_t oracle_tr_size(void) { return sizeof(OTranscript); }
+
 /* goldilocks_tooling.hpp:9-21 */
 uint64_t oracle_tree_num_elements(uint64_t rows, uint64_t arity)
 {

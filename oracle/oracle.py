@@ -77,6 +77,12 @@ class Oracle:
         L.oracle_p2_linear_hash.argtypes = [_u64p, _u64p, _u64, _u64, _u32]
         L.oracle_tree_num_elements.restype = _u64
         L.oracle_tree_num_elements.argtypes = [_u64, _u64]
+        L.oracle_tr_size.restype = _u64
+        L.oracle_tr_init.argtypes = [ctypes.c_void_p, _u32]
+        L.oracle_tr_put.argtypes = [ctypes.c_void_p, _u64p, _u64]
+        L.oracle_tr_get_field.argtypes = [ctypes.c_void_p, _u64p]
+        L.oracle_tr_get_state.argtypes = [ctypes.c_void_p, _u64p, _u32]
+        L.oracle_tr_get_permutations.argtypes = [ctypes.c_void_p, _u64p, _u64, _u64]
         L.oracle_p2_grinding.restype = _u64
         L.oracle_p2_grinding.argtypes = [_u64p, _u32, _u64]
         L.oracle_p2_merkletree.argtypes = [_u64p, _u64p, _u64, _u64, _u64, _u64, _u64]
@@ -145,6 +151,9 @@ class Oracle:
         self.L.oracle_p2_linear_hash(_ptr(out), _ptr(v) if v.size else _ptr(np.zeros(1, np.uint64)), v.size, 1, width)
         return out
 
+    def transcript(self, arity):
+        return OracleTranscript(self.L, arity)
+
     def grinding(self, challenge, n_bits, limit=1 << 24):
         ch = np.ascontiguousarray(np.asarray(challenge, dtype=np.uint64)[:3])
         return int(self.L.oracle_p2_grinding(_ptr(ch), n_bits, limit))
@@ -196,6 +205,35 @@ class Oracle:
 
     def num_threads(self):
         return self.L.oracle_num_threads()
+
+
+class OracleTranscript:
+    """TranscriptGL restated in C (oracle_tr_*)."""
+
+    def __init__(self, L, arity):
+        self.L = L
+        self.buf = ctypes.create_string_buffer(int(L.oracle_tr_size()))
+        L.oracle_tr_init(self.buf, arity)
+
+    def put(self, vals):
+        v = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64))
+        if v.size:
+            self.L.oracle_tr_put(self.buf, _ptr(v), v.size)
+
+    def get_field(self):
+        out = np.zeros(3, dtype=np.uint64)
+        self.L.oracle_tr_get_field(self.buf, _ptr(out))
+        return out
+
+    def get_state(self, n=4):
+        out = np.zeros(n, dtype=np.uint64)
+        self.L.oracle_tr_get_state(self.buf, _ptr(out), n)
+        return out
+
+    def get_permutations(self, n, n_bits):
+        out = np.zeros(n, dtype=np.uint64)
+        self.L.oracle_tr_get_permutations(self.buf, _ptr(out), n, n_bits)
+        return out
 
 
 class RefLib:

@@ -9,11 +9,19 @@
 #define P2_COOP_MAX_PARENTS 16384
 #endif
 
+// Transcript sponge state (device memory), see k_transcript.
+struct P2TranscriptState {
+    uint64_t state[16], pending[12], out[16];
+    uint32_t pending_cursor, out_cursor;
+};
+
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st);
 extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st);
+extern "C" int p2_launch_transcript(uint32_t width, P2TranscriptState *t, uint64_t *io, uint64_t n, int op,
+                                    hipStream_t st);
 extern "C" int p2_launch_grind(const uint64_t *ch, uint64_t level, uint64_t base, uint64_t count,
                                uint64_t *found, hipStream_t st);
 extern "C" int p2_launch_group_proofs(const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols, const uint64_t *tree,

@@ -404,6 +404,64 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_grind(uint64_t c0, uint64_t c
     if (gl_canon(s[0]) < level) atomicMin(found, (unsigned long long)(base + k));
 }
 
+// --------------------------------------------------------- transcript ----
+// Fiat-Shamir transcript (TranscriptGL, transcriptGL.cpp:4-119): a sponge of
+// width W = 4*arity, rate W-4, capacity 4. Inputs collect in `pending`; a full
+// rate (or a read with inputs pending) permutes [pending, zero-padded |
+// state[0..4)], and the whole output becomes the new state and the output
+// buffer that field reads consume from the front. One lane runs it: the work is
+// a few permutations per proof, so latency is all that matters.
+template <int W>
+__device__ void tr_update(P2TranscriptState *t)
+{
+    constexpr int RATE = W - 4;
+    uint64_t s[W];
+#pragma unroll
+    for (int i = 0; i < RATE; i++) s[i] = (uint32_t)i < t->pending_cursor ? t->pending[i] : 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) s[RATE + i] = t->state[i];
+    p2_permute<W>(s);
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t v = gl_canon(s[i]);
+        t->out[i] = v;
+        t->state[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < RATE; i++) t->pending[i] = 0;
+    t->pending_cursor = 0;
+    t->out_cursor = W;
+}
+
+template <int W>
+__device__ uint64_t tr_field1(P2TranscriptState *t)
+{
+    if (t->out_cursor == 0) tr_update<W>(t);
+    uint64_t r = t->out[(W - t->out_cursor) % W];
+    t->out_cursor--;
+    return r;
+}
+
+// op 0: absorb n words from io; op 1: squeeze n field words into io;
+// op 2: the first n state words into io (after absorbing pending inputs)
+template <int W>
+__global__ void k_transcript(P2TranscriptState *t, uint64_t *io, uint64_t n, int op)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (op == 0) {
+        for (uint64_t k = 0; k < n; k++) {
+            t->pending[t->pending_cursor++] = io[k];
+            t->out_cursor = 0;
+            if (t->pending_cursor == W - 4) tr_update<W>(t);
+        }
+    } else if (op == 1) {
+        for (uint64_t k = 0; k < n; k++) io[k] = tr_field1<W>(t);
+    } else {
+        if (t->pending_cursor > 0) tr_update<W>(t);
+        for (uint64_t k = 0; k < n; k++) io[k] = t->state[k];
+    }
+}
+
 // Single permutation over `count` states stored consecutively (W words each).
 template <int W>
 __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
@@ -480,6 +538,18 @@ extern "C" int p2_launch_grind(const uint64_t *ch, uint64_t level, uint64_t base
     dim3 g((unsigned)((count + P2_THREADS - 1) / P2_THREADS));
     hipLaunchKernelGGL(k_p2_grind, g, dim3(P2_THREADS), 0, st, ch[0], ch[1], ch[2], level, base, count,
                        (unsigned long long *)found);
+    return (int)hipGetLastError();
+}
+
+extern "C" int p2_launch_transcript(uint32_t width, P2TranscriptState *t, uint64_t *io, uint64_t n, int op,
+                                    hipStream_t st)
+{
+    switch (width) {
+    case 8: hipLaunchKernelGGL(k_transcript<8>, dim3(1), dim3(64), 0, st, t, io, n, op); break;
+    case 12: hipLaunchKernelGGL(k_transcript<12>, dim3(1), dim3(64), 0, st, t, io, n, op); break;
+    case 16: hipLaunchKernelGGL(k_transcript<16>, dim3(1), dim3(64), 0, st, t, io, n, op); break;
+    default: return -1;
+    }
     return (int)hipGetLastError();
 }
 

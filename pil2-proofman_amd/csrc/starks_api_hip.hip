@@ -551,6 +551,132 @@ int64_t pil2hip_grinding(uint64_t *nonce, const uint64_t *challenge, uint32_t nB
     return fail(-37, "grinding: no valid nonce below the bound");
 }
 
+// ------------------------------------------------------------- transcript --
+struct TranscriptHip {
+    P2TranscriptState *d = nullptr;  // device state
+    uint64_t *d_io = nullptr;        // device staging for inputs and outputs
+    uint64_t io_words = 0;
+    uint32_t width = 0;
+};
+
+static int64_t tr_io(TranscriptHip *t, uint64_t n)
+{
+    if (n <= t->io_words) return 0;
+    if (t->d_io) (void)hipFree(t->d_io);
+    t->d_io = nullptr;
+    t->io_words = 0;
+    uint64_t w = n < 64 ? 64 : n;
+    HIP_TRY(hipMalloc(&t->d_io, 8 * w), "hipMalloc(transcript io)");
+    t->io_words = w;
+    return 0;
+}
+
+void *pil2hip_transcript_create(uint64_t arity)
+{
+    if (check_arity(arity)) return nullptr;
+    TranscriptHip *t = new TranscriptHip();
+    t->width = (uint32_t)(4 * arity);
+    P2TranscriptState zero;
+    memset(&zero, 0, sizeof(zero));
+    if (hipMalloc(&t->d, sizeof(P2TranscriptState)) != hipSuccess ||
+        hipMemcpy(t->d, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess || tr_io(t, 64)) {
+        if (t->d) (void)hipFree(t->d);
+        delete t;
+        fail(-8, "transcript: device allocation failed");
+        return nullptr;
+    }
+    return t;
+}
+
+void pil2hip_transcript_destroy(void *handle)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t) return;
+    (void)hipDeviceSynchronize();
+    if (t->d) (void)hipFree(t->d);
+    if (t->d_io) (void)hipFree(t->d_io);
+    delete t;
+}
+
+static int64_t tr_run(TranscriptHip *t, uint64_t *io, uint64_t n, int op, hipStream_t st)
+{
+    int e = p2_launch_transcript(t->width, t->d, io, n, op, st);
+    if (e) return fail(-100 - e, "transcript launch failed");
+    return 0;
+}
+
+int64_t pil2hip_transcript_put(void *handle, const uint64_t *d_in, uint64_t n, void *stream)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t || (n && !d_in)) return fail(-13, "null argument");
+    if (n == 0) return 0;
+    return tr_run(t, (uint64_t *)d_in, n, 0, (hipStream_t)stream);
+}
+
+int64_t pil2hip_transcript_put_host(void *handle, const uint64_t *h_in, uint64_t n, void *stream)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t || (n && !h_in)) return fail(-13, "null argument");
+    if (n == 0) return 0;
+    int64_t rc = tr_io(t, n);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(t->d_io, h_in, 8 * n, hipMemcpyHostToDevice, st), "transcript input copy");
+    if ((rc = tr_run(t, t->d_io, n, 0, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(st), "transcript sync");  // the staging buffer is reused
+    return 0;
+}
+
+static int64_t tr_read(TranscriptHip *t, uint64_t *h_out, uint64_t n, int op, hipStream_t st)
+{
+    int64_t rc = tr_io(t, n);
+    if (rc) return rc;
+    if ((rc = tr_run(t, t->d_io, n, op, st))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, t->d_io, 8 * n, hipMemcpyDeviceToHost, st), "transcript output copy");
+    HIP_TRY(hipStreamSynchronize(st), "transcript sync");
+    return 0;
+}
+
+int64_t pil2hip_transcript_get_field(void *handle, uint64_t *h_out, void *stream)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t || !h_out) return fail(-13, "null argument");
+    return tr_read(t, h_out, 3, 1, (hipStream_t)stream);
+}
+
+int64_t pil2hip_transcript_get_state(void *handle, uint64_t *h_out, uint64_t n, void *stream)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t || !h_out) return fail(-13, "null argument");
+    if (n > t->width) return fail(-38, "transcript state has 4*arity words");
+    return tr_read(t, h_out, n, 2, (hipStream_t)stream);
+}
+
+int64_t pil2hip_transcript_get_permutations(void *handle, uint64_t *h_out, uint64_t n, uint64_t nBits, void *stream)
+{
+    TranscriptHip *t = (TranscriptHip *)handle;
+    if (!t || (n && !h_out)) return fail(-13, "null argument");
+    if (n == 0) return 0;
+    if (nBits < 1 || nBits > 63) return fail(-39, "permutation bits must be 1..63");
+    uint64_t nfields = (n * nBits - 1) / 63 + 1;
+    std::vector<uint64_t> f(nfields);
+    int64_t rc = tr_read(t, f.data(), nfields, 1, (hipStream_t)stream);
+    if (rc) return rc;
+    uint64_t cur = 0, bit = 0;  // 63 bits per field, lowest first (transcriptGL.cpp:92-118)
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t a = 0;
+        for (uint64_t j = 0; j < nBits; j++) {
+            if ((f[cur] >> bit) & 1) a |= 1ull << j;
+            if (++bit == 63) {
+                bit = 0;
+                cur++;
+            }
+        }
+        h_out[i] = a;
+    }
+    return 0;
+}
+
 void pil2hip_timing_enable(bool on) { g_timing = on; }
 
 uint64_t pil2hip_timing_read(double *phase_ms, bool reset)

@@ -53,6 +53,13 @@ _SIGS = {
     "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
     "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
     "pil2hip_grinding": (_i64, [_vp, _vp, ctypes.c_uint32, _vp]),
+    "pil2hip_transcript_create": (_vp, [_u64]),
+    "pil2hip_transcript_put": (_i64, [_vp, _vp, _u64, _vp]),
+    "pil2hip_transcript_put_host": (_i64, [_vp, _vp, _u64, _vp]),
+    "pil2hip_transcript_get_field": (_i64, [_vp, _vp, _vp]),
+    "pil2hip_transcript_get_state": (_i64, [_vp, _vp, _u64, _vp]),
+    "pil2hip_transcript_get_permutations": (_i64, [_vp, _vp, _u64, _u64, _vp]),
+    "pil2hip_transcript_destroy": (None, [_vp]),
     "pil2hip_proof_words": (_u64, [_u64, _u64, _u64, _u64]),
     "pil2hip_serve_create": (_vp, [_vp]),
     "pil2hip_serve_submit": (_i64, [_vp, _vp]),
@@ -535,3 +542,49 @@ def grinding(challenge, n_bits, stream=None):
     out = np.zeros(1, dtype=np.uint64)
     _check(_lib.pil2hip_grinding(out.ctypes.data_as(_vp), ch.ctypes.data_as(_vp), n_bits, _stream(stream)))
     return int(out[0])
+
+
+class TranscriptGL:
+    """Mirror of TranscriptGL (transcriptGL.hpp:11-58) with the sponge state in
+    device memory (pil2hip_transcript_*)."""
+
+    def __init__(self, arity, custom=False):
+        self.h = _lib.pil2hip_transcript_create(arity)
+        if not self.h:
+            raise HipError("pil2hip_transcript_create: " + _lib.pil2hip_last_error().decode())
+
+    def put(self, values, stream=None):
+        """values: numpy/list (host) or a device tensor."""
+        if hasattr(values, "data_ptr"):
+            _check(_lib.pil2hip_transcript_put(self.h, _ptr(values), values.numel(), _stream(stream)))
+        else:
+            v = np.ascontiguousarray(np.asarray(values, dtype=np.uint64))
+            if v.size:
+                _check(_lib.pil2hip_transcript_put_host(self.h, v.ctypes.data_as(_vp), v.size, _stream(stream)))
+
+    def get_field(self, stream=None):
+        out = np.zeros(3, dtype=np.uint64)
+        _check(_lib.pil2hip_transcript_get_field(self.h, out.ctypes.data_as(_vp), _stream(stream)))
+        return out
+
+    def get_state(self, n=4, stream=None):
+        out = np.zeros(n, dtype=np.uint64)
+        _check(_lib.pil2hip_transcript_get_state(self.h, out.ctypes.data_as(_vp), n, _stream(stream)))
+        return out
+
+    def get_permutations(self, n, n_bits, stream=None):
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        _check(_lib.pil2hip_transcript_get_permutations(self.h, out.ctypes.data_as(_vp), n, n_bits,
+                                                        _stream(stream)))
+        return out[:n]
+
+    def close(self):
+        if self.h:
+            _lib.pil2hip_transcript_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -383,3 +383,30 @@ def test_grinding_large_matches_compiled_reference(pc):
         pytest.skip("oracle/_ref not built")
     ch = [0x0123456789ABCDEF, 42, 2**63 + 5]
     assert pc.grinding(ch, 20) == RefLib().grinding(ch, 20)
+
+
+# ------------------------------------------------------------ transcript --
+@pytest.mark.parametrize("arity", [2, 3, 4])
+def test_transcript_matches_oracle(pc, oracle, arity):
+    """Device transcript (TranscriptGL) against the oracle on seeded sequences of
+    puts, field draws, state reads and query-index draws."""
+    from test_oracle import run_transcript, transcript_ops
+    for seed in range(3):
+        ops = transcript_ops(seed * 10 + arity, 4 * arity)
+        t = pc.TranscriptGL(arity)
+        assert run_transcript(t, ops) == run_transcript(oracle.transcript(arity), ops), seed
+        t.close()
+
+
+def test_transcript_absorbs_device_root(pc, oracle):
+    """put() of a Merkle root that stays in HBM equals put() of its host copy."""
+    nb, c = 8, 5
+    trace = datagen.uniform((1 << nb) * c, 77).reshape(1 << nb, c)
+    lde, nodes = pc.commit(pc.to_device(trace), 3, nb, nb + 1, c)
+    root_dev = nodes[-4:]
+    t = pc.TranscriptGL(3)
+    t.put(root_dev)
+    got = t.get_field()
+    o = oracle.transcript(3)
+    o.put(oracle.merkletree(oracle.lde(trace, 1 << (nb + 1)), 3)[-4:])
+    assert (got == o.get_field()).all()

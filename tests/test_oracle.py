@@ -292,3 +292,101 @@ def test_grinding_matches_compiled_reference(oracle):
     ref = RefLib()
     for ch, nb in GRIND_CASES:
         assert ref.grinding(ch, nb) == oracle.grinding(ch, nb), (ch, nb)
+
+
+# ------------------------------------------------------------ transcript --
+class RustSpecTranscript:
+    """fields/src/transcript.rs:24-124 (the Rust twin of TranscriptGL) restated in
+    Python over the oracle's permutation; an independent spec for the C oracle."""
+
+    def __init__(self, oracle, width):
+        self.o, self.W, self.RATE = oracle, width, width - 4
+        self.state, self.out = [0] * width, [0] * width
+        self.pending, self.pending_cursor, self.out_cursor = [0] * self.RATE, 0, 0
+
+    def update_state(self):
+        while self.pending_cursor < self.RATE:
+            self.pending[self.pending_cursor] = 0
+            self.pending_cursor += 1
+        inputs = self.pending[:] + self.state[:4]
+        h = [int(x) % P for x in self.o.permute(inputs)]
+        self.out_cursor = self.W
+        self.pending, self.pending_cursor = [0] * self.RATE, 0
+        self.state, self.out = h[:], h[:]
+
+    def put(self, vals):
+        for v in vals:
+            self.pending[self.pending_cursor] = int(v)
+            self.pending_cursor += 1
+            self.out_cursor = 0
+            if self.pending_cursor == self.RATE:
+                self.update_state()
+
+    def get_state(self, n=4):
+        if self.pending_cursor > 0:
+            self.update_state()
+        return self.state[:n]
+
+    def get_fields1(self):
+        if self.out_cursor == 0:
+            self.update_state()
+        val = self.out[(self.W - self.out_cursor) % self.W]
+        self.out_cursor -= 1
+        return val
+
+    def get_field(self):
+        return [self.get_fields1() for _ in range(3)]
+
+    def get_permutations(self, n, n_bits):
+        fields = [self.get_fields1() % P for _ in range((n * n_bits - 1) // 63 + 1)]
+        cur, bit, res = 0, 0, []
+        for _ in range(n):
+            a = 0
+            for j in range(n_bits):
+                if (fields[cur] >> bit) & 1:
+                    a += 1 << j
+                bit += 1
+                if bit == 63:
+                    bit, cur = 0, cur + 1
+            res.append(a)
+        return res
+
+
+def transcript_ops(seed, width, n_ops=60):
+    """A seeded sequence of transcript operations (puts of 0..30 words, some >= p)."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(n_ops):
+        k = int(rng.integers(0, 4))
+        if k == 0:
+            vals = rng.integers(0, 2**64, int(rng.integers(0, 31)), dtype=np.uint64)
+            ops.append(("put", vals))
+        elif k == 1:
+            ops.append(("field", None))
+        elif k == 2:
+            ops.append(("state", int(rng.integers(1, width + 1))))
+        else:
+            ops.append(("perms", (int(rng.integers(1, 41)), int(rng.integers(1, 31)))))
+    return ops
+
+
+def run_transcript(t, ops):
+    out = []
+    for op, arg in ops:
+        if op == "put":
+            t.put(arg)
+        elif op == "field":
+            out.append([int(x) for x in t.get_field()])
+        elif op == "state":
+            out.append([int(x) for x in t.get_state(arg)])
+        else:
+            out.append([int(x) for x in t.get_permutations(*arg)])
+    return out
+
+
+@pytest.mark.parametrize("arity", [2, 3, 4])
+def test_transcript_oracle_matches_rust_spec(oracle, arity):
+    for seed in range(4):
+        ops = transcript_ops(seed * 10 + arity, 4 * arity)
+        assert run_transcript(oracle.transcript(arity), ops) == run_transcript(RustSpecTranscript(oracle, 4 * arity),
+                                                                                ops), seed
