@@ -87,6 +87,22 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
 
+/* Starks::computeQ (starks.hpp:187-239; GPU twin NTTGoldilocksGPU::computeQ,
+ * ntt_goldilocks.cu:1583-1604): the quotient q, qDim columns over the extended
+ * coset (element (k, d) at d_q[k*qRowStride + d*qColStride]; column-major with
+ * qRowStride 1 and qColStride >= NExt, or row-major with qColStride 1), is
+ * interpolated over NExt points; coefficient p*N + i of column d times
+ * shiftIn^p (shiftIn = 7^-N) becomes coefficient i of column p*qDim + d, and
+ * each of the qDeg*qDim columns is zero-extended and transformed over NExt
+ * points into d_cmQ (column-major, column c at d_cmQ + c*NExt). d_q is not
+ * modified and must not overlap d_cmQ. d_scratch: >= qDeg*qDim*2^nBits words.
+ * With d_nodes != NULL the rows of d_cmQ are then merkelized (the step's
+ * merkelize, starks.hpp:228-235); the root is read with pil2hip_get_root.
+ * 1 <= qDeg <= 2^(nBitsExt - nBits), nBitsExt <= 24. */
+int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_q, uint64_t qRowStride,
+                          uint64_t qColStride, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt, uint64_t qDeg,
+                          uint64_t qDim, uint64_t arity, void *stream);
+
 /* One commit of a batch: the arguments of pil2hip_commit. Buffers are device
  * memory owned by the caller and must not overlap between jobs. */
 typedef struct Pil2CommitJob {

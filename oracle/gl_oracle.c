@@ -220,6 +220,45 @@ void oracle_lde(uint64_t *out, const uint64_t *in, uint64_t NExt, uint64_t N, ui
     free(shiftpow);
 }
 
+/* Starks::computeQ, pil2-stark/src/starkpil/starks.hpp:187-239. q: row-major
+ * NExt x qDim (the extended-coset evaluations); each column is interpolated over
+ * <w_NExt> (the reference's in-place INTT, :196-200), coefficient p*N + i of
+ * column d times S[p] = shiftIn^p, shiftIn = (1/7)^N (:202-207), lands in row i,
+ * column p*qDim + d of cmQ for i < N (:209-216); rows N..NExt are zero
+ * (:218-221) and every cmQ column is transformed over NExt points (:222-226).
+ * cmq: row-major NExt x (qDeg*qDim). q is not modified. */
+void oracle_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t N, uint64_t NExt, uint64_t qDeg, uint64_t qDim)
+{
+    if (!N || !NExt || !qDeg || !qDim) return;
+    uint32_t lE = o_log2(NExt);
+    uint64_t nc = qDeg * qDim;
+    uint64_t shift_in = o_pow(o_inv(7), N);
+    uint64_t *S = (uint64_t *)malloc(sizeof(uint64_t) * qDeg);
+    S[0] = 1;
+    for (uint64_t p = 1; p < qDeg; p++) S[p] = o_mul(S[p - 1], shift_in);
+    uint64_t winv = o_inv(O_W[lE]), ninv = o_inv(NExt % GLP);
+    #pragma omp parallel
+    {
+        uint64_t *col = (uint64_t *)malloc(sizeof(uint64_t) * NExt);
+        uint64_t *out = (uint64_t *)malloc(sizeof(uint64_t) * NExt);
+        #pragma omp for schedule(dynamic, 1)
+        for (uint64_t d = 0; d < qDim; d++) {
+            for (uint64_t k = 0; k < NExt; k++) col[k] = q[k * qDim + d];
+            o_dft_inplace(col, lE, winv);
+            for (uint64_t k = 0; k < NExt; k++) col[k] = o_mul(col[k], ninv);
+            for (uint64_t p = 0; p < qDeg; p++) {
+                for (uint64_t i = 0; i < N; i++) out[i] = o_mul(col[p * N + i], S[p]);
+                for (uint64_t i = N; i < NExt; i++) out[i] = 0;
+                o_dft_inplace(out, lE, O_W[lE]);
+                for (uint64_t j = 0; j < NExt; j++) cmq[j * nc + p * qDim + d] = out[j];
+            }
+        }
+        free(col);
+        free(out);
+    }
+    free(S);
+}
+
 /* ------------------------------------------------------------- Poseidon2 -- */
 static const uint64_t *o_rc(uint32_t W) { return W == 8 ? P2_RC8 : W == 12 ? P2_RC12 : P2_RC16; }
 static const uint64_t *o_diag(uint32_t W) { return W == 8 ? P2_DIAG8 : W == 12 ? P2_DIAG12 : P2_DIAG16; }

@@ -73,6 +73,7 @@ class Oracle:
         L.oracle_poly_eval.restype = _u64
         L.oracle_poly_eval.argtypes = [_u64p, _u64, _u64]
         L.oracle_lde.argtypes = [_u64p, _u64p, _u64, _u64, _u64]
+        L.oracle_compute_q.argtypes = [_u64p, _u64p, _u64, _u64, _u64, _u64]
         L.oracle_p2_permute.argtypes = [_u64p, _u32]
         L.oracle_p2_linear_hash.argtypes = [_u64p, _u64p, _u64, _u64, _u32]
         L.oracle_tree_num_elements.restype = _u64
@@ -137,6 +138,14 @@ class Oracle:
         n, ncols = trace.shape
         out = np.zeros((n_ext, ncols), dtype=np.uint64)
         self.L.oracle_lde(_ptr(out), _ptr(trace), n_ext, n, ncols)
+        return out
+
+    def compute_q(self, q, n, q_deg):
+        """q: row-major (NExt, qDim) -> cmQ row-major (NExt, q_deg*qDim)."""
+        q = np.ascontiguousarray(q, dtype=np.uint64)
+        n_ext, q_dim = q.shape
+        out = np.zeros((n_ext, q_deg * q_dim), dtype=np.uint64)
+        self.L.oracle_compute_q(_ptr(out), _ptr(q), n, n_ext, q_deg, q_dim)
         return out
 
     # Poseidon2 ---------------------------------------------------------
@@ -249,6 +258,7 @@ class RefLib:
         L = ctypes.CDLL(path)
         self.L = L
         L.ref_lde.argtypes = [_u64p, _u64p, _u64, _u64, _u64]
+        L.ref_ntt.argtypes = [_u64p, _u64, _u64, ctypes.c_int]
         L.ref_merkletree.restype = ctypes.c_int
         L.ref_merkletree.argtypes = [_u64p, _u64p, _u64, _u64, _u64]
         L.ref_permute.restype = ctypes.c_int
@@ -271,6 +281,14 @@ class RefLib:
         out = np.zeros((n_ext, ncols), dtype=np.uint64)
         self.L.ref_lde(_ptr(out), _ptr(trace), n_ext, n, ncols)
         return out % np.uint64(P) if out.size else out
+
+    def ntt(self, a, inverse=False):
+        """NTT_Goldilocks::NTT / INTT on a row-major (size, ncols) array."""
+        a = np.array(a, dtype=np.uint64)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        self.L.ref_ntt(_ptr(a), a.shape[0], a.shape[1], int(inverse))
+        return a % np.uint64(P)
 
     def merkletree(self, src, arity, tree_len):
         # The reference's batch leaf/node loops work 8 rows at a time and touch

@@ -7,6 +7,7 @@
 //
 // Each wrapper names the reference entry point it calls:
 //   ref_lde        -> NTT_Goldilocks(N).LDE          ntt_goldilocks.cpp:369-404
+//   ref_ntt        -> NTT_Goldilocks(n).NTT / INTT   ntt_goldilocks.cpp:211-260, ntt_goldilocks.hpp:187-191
 //   ref_merkletree -> Poseidon2Goldilocks<4a>::merkletree (Auto mode, as
 //                     MerkleTreeGL::merkelize does)  merkleTreeGL.cpp:316-336
 //   ref_permute    -> Poseidon2Goldilocks<W>::permute poseidon2_goldilocks.hpp:252-272
@@ -34,6 +35,17 @@ void ref_lde(uint64_t *out, const uint64_t *in, uint64_t NExt, uint64_t N, uint6
 {
     NTT_Goldilocks ntt(N);
     ntt.LDE((Goldilocks::Element *)out, (Goldilocks::Element *)in, NExt, N, ncols);
+}
+
+// in place on a row-major size x ncols array
+void ref_ntt(uint64_t *a, uint64_t size, uint64_t ncols, int inverse)
+{
+    NTT_Goldilocks ntt(size);
+    Goldilocks::Element *x = (Goldilocks::Element *)a;
+    if (inverse)
+        ntt.INTT(x, x, size, ncols);
+    else
+        ntt.NTT(x, x, size, ncols);
 }
 
 int ref_merkletree(uint64_t *tree, const uint64_t *src, uint64_t ncols, uint64_t nrows, uint64_t arity)

@@ -17,6 +17,12 @@ struct NttTablesHost {
 extern "C" int ntt_init_device(void);
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
                               uint64_t ncols, const NttTablesHost *th, hipStream_t st, hipEvent_t *ev);
+// computeQ (INTT over NExt, x shiftIn^p, regroup, zero-extend, NTT over NExt): q element
+// (k, d) at q[k*q_rs + d*q_cs] (q_rs == 1: column-major; else q_cs == 1), cmq column-major
+// (q_deg*q_dim columns of NExt), coef scratch >= q_deg*q_dim*N words.
+extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q_rs, uint64_t q_cs, uint64_t *coef,
+                                    uint32_t log_n, uint32_t log_ext, uint32_t q_deg, uint32_t q_dim,
+                                    const NttTablesHost *th, hipStream_t st);
 extern "C" int ntt_launch_cm2rm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_cs,
                                 uint64_t out_rs, hipStream_t st);
 extern "C" int ntt_launch_rm2cm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_rs,

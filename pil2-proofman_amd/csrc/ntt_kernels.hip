@@ -410,7 +410,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
 
 template <int V>
 __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t m, uint32_t L2, uint32_t log_n,
-                                             uint64_t n_inv, const NttTables &tb)
+                                             uint64_t n_inv, const NttTables &tb, bool scale = true)
 {
     constexpr int LV = Log2<V>::v;
     bool active = threadIdx.x < ((1u << L2) >> LV);
@@ -418,7 +418,7 @@ __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t 
     while (q > 0) {
         int r = q >= LV ? LV : q;
         q -= r;
-        bool last = (q == 0);
+        bool last = (q == 0) && scale;
         if (active) {
             switch (r) {
             case 4: if constexpr (V >= 16) block_dif_round<16, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
@@ -483,6 +483,10 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
     }
 }
 
+// COEF_IN: the line already holds coefficients in phase 1's output order
+// (position k = coefficient bitrev_{L1+L2}(m*n2 + k), scaled); phase 1 is skipped
+// and the block only extends (computeQ's second half).
+template <bool COEF_IN>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
                                                            uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
                                                            uint32_t L2, uint32_t logb, uint64_t n_inv, NttTables tb)
@@ -494,11 +498,13 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
     // phase 1: load the line, iNTT DIF over L2 bits with 16/B values per thread
     for (uint32_t k = threadIdx.x; k < (1u << L2); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
     __syncthreads();
-    switch (logb) {
-    case 0: block_phase1<16>(s, m, L2, log_n, n_inv, tb); break;
-    case 1: block_phase1<8>(s, m, L2, log_n, n_inv, tb); break;
-    case 2: block_phase1<4>(s, m, L2, log_n, n_inv, tb); break;
-    default: block_phase1<2>(s, m, L2, log_n, n_inv, tb); break;
+    if constexpr (!COEF_IN) {
+        switch (logb) {
+        case 0: block_phase1<16>(s, m, L2, log_n, n_inv, tb); break;
+        case 1: block_phase1<8>(s, m, L2, log_n, n_inv, tb); break;
+        case 2: block_phase1<4>(s, m, L2, log_n, n_inv, tb); break;
+        default: block_phase1<2>(s, m, L2, log_n, n_inv, tb); break;
+        }
     }
     bool first = true;
     // phase 2: NTT DIT over bits [logb, logb + L2) of the n2*B spread values
@@ -547,6 +553,70 @@ __global__ __launch_bounds__(256) void k_lde_tiny(const uint64_t *__restrict__ i
     out[(uint64_t)blockIdx.y * cs_out + j] = acc;
 }
 
+// ------------------------------------------------------------ computeQ ----
+// Semantics of Starks::computeQ (starks.hpp:187-239): q (qDim columns over the
+// extended coset) is interpolated over NExt points, coefficient p*N + i of column
+// d times shiftIn^p = 7^(-p*N) becomes coefficient i of column p*qDim + d
+// (p < qDeg, i < N), and every such column is zero-extended and transformed
+// over NExt points.
+//
+// Plan (NExt = n1 * n2*B, the LDE's split): after the strided inverse pass over
+// n1, block m of a q column holds n2*B values; their DIF leaves at tile position
+// x = (k << logb) | b the coefficient bitrev_{log NExt}(m*n2*B + x) = p*N + i with
+// p = bitrev_logb(b) and i = bitrev_{log N}(m*n2 + k). So block m holds, for every
+// p, exactly the n2 coefficients that block m of an N -> NExt LDE of column
+// p*qDim + d works on, in the order that block expects: k_q_block_inv writes
+// them (times shiftIn^p / NExt) to line m of coefficient column p*qDim + d, and
+// k_lde_block<true> plus the strided forward pass finish the extension.
+__global__ __launch_bounds__(NTT_THREADS) void k_q_block_inv(const uint64_t *__restrict__ in, uint64_t cs_in,
+                                                             uint64_t *__restrict__ coef, uint64_t cs_coef, uint32_t L1,
+                                                             uint32_t L2, uint32_t logb, uint32_t q_deg, uint32_t q_dim,
+                                                             uint64_t n_inv_ext, uint64_t shift_in, NttTables tb)
+{
+    extern __shared__ uint64_t s[];
+    uint32_t m = blockIdx.x, d = blockIdx.y;
+    uint32_t lb = L2 + logb;
+    const uint64_t *src = in + (uint64_t)d * cs_in + ((uint64_t)m << lb);
+    for (uint32_t k = threadIdx.x; k < (1u << lb); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
+    __syncthreads();
+    block_phase1<16>(s, m, lb, L1 + lb, 0, tb, false);
+    uint32_t n2 = 1u << L2;
+    uint64_t sc = n_inv_ext;  // shiftIn^p / NExt
+    for (uint32_t p = 0; p < q_deg; p++) {
+        uint64_t *dst = coef + (uint64_t)(p * q_dim + d) * cs_coef + ((uint64_t)m << L2);
+        uint32_t b = bitrev32(p, logb);
+        for (uint32_t k = threadIdx.x; k < n2; k += blockDim.x) dst[k] = gl_mul(s[lds_addr((k << logb) | b)], sc);
+        sc = gl_mul(sc, shift_in);
+    }
+}
+
+// NExt < 16 or N == 1: one lane per output row j of column c = p*qDim + d, by the
+// definition: a_i = shiftIn^p / NExt * sum_k q_d[k] w^-((p*N + i)*k), then
+// out_j = sum_{i<N} a_i w^(i*j), w = w_NExt. q element (k, d) at q[k*rs + d*cs].
+__global__ __launch_bounds__(256) void k_q_tiny(const uint64_t *__restrict__ q, uint64_t rs, uint64_t cs,
+                                                uint64_t *__restrict__ out, uint32_t log_n, uint32_t log_ext,
+                                                uint32_t q_dim, uint64_t n_inv_ext, uint64_t shift_in, uint64_t w_inv,
+                                                uint64_t w)
+{
+    uint32_t n = 1u << log_n, ne = 1u << log_ext;
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    uint32_t c = blockIdx.y, p = c / q_dim, d = c - p * q_dim;
+    uint64_t sc = n_inv_ext;
+    for (uint32_t t = 0; t < p; t++) sc = gl_mul(sc, shift_in);
+    uint64_t wj = gl_pow(w, j), acc = 0, xp = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        uint64_t wk = gl_pow(w_inv, p * n + i), a = 0, wp = 1;
+        for (uint32_t k = 0; k < ne; k++) {
+            a = gl_add(a, gl_mul(gl_canon(q[(uint64_t)k * rs + (uint64_t)d * cs]), wp));
+            wp = gl_mul(wp, wk);
+        }
+        acc = gl_add(acc, gl_mul(gl_mul(a, sc), xp));
+        xp = gl_mul(xp, wj);
+    }
+    out[(uint64_t)c * ne + j] = acc;
+}
+
 // ------------------------------------------------------------ launchers ----
 static const uint64_t kW[33] = {
     0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
@@ -582,7 +652,11 @@ extern "C" int ntt_init_device(void)
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_lde_block, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+            e = hipFuncSetAttribute((const void *)k_lde_block<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_lde_block<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_q_block_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return (int)e;
         g_attr_done = true;
     }
@@ -635,7 +709,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
     {
         uint32_t log_tile = L2 + log_b;
         dim3 g((unsigned)(1u << L1), (unsigned)ncols);
-        hipLaunchKernelGGL(k_lde_block, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
+        hipLaunchKernelGGL(k_lde_block<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
                            N << log_b, L1, L2, log_b, n_inv, tb);
     }
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -650,6 +724,68 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
                            N << log_b, L1, l2b, lt, tb);
     }
     if (ev) (void)hipEventRecord(ev[3], st);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q_rs, uint64_t q_cs, uint64_t *coef,
+                                    uint32_t log_n, uint32_t log_ext, uint32_t q_deg, uint32_t q_dim,
+                                    const NttTablesHost *th, hipStream_t st)
+{
+    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
+    uint32_t log_b = log_ext - log_n;
+    if (q_deg == 0 || q_dim == 0 || q_deg > (1u << log_b)) return -3;
+    uint64_t N = 1ull << log_n, NExt = 1ull << log_ext;
+    uint32_t ncols = q_deg * q_dim;
+    uint64_t n_inv_ext = gl_inv(NExt % GL_P);
+    uint64_t shift_in = gl_pow(gl_inv(7), N);  // Goldilocks::shift() = 7
+    if (log_n == 0 || log_ext < 4) {
+        if (log_ext > 12) return -2;
+        hipLaunchKernelGGL(k_q_tiny, dim3((unsigned)((NExt + 255) / 256), ncols), dim3(256), 0, st, q, q_rs, q_cs, cmq,
+                           log_n, log_ext, q_dim, n_inv_ext, shift_in, gl_inv(kW[log_ext]), kW[log_ext]);
+        return (int)hipGetLastError();
+    }
+    if (log_b > NTT_LOG_TILE) return -2;
+    // the LDE's split: n2*B <= 2^14 values per block, n1 = N / n2 <= 2^10
+    uint32_t L2 = log_n;
+    if (L2 + log_b > NTT_LOG_TILE) L2 = NTT_LOG_TILE - log_b;
+    uint32_t L1 = log_n - L2, lb = L2 + log_b;
+    if (L1 > 10 || L2 < 1 || lb < 4) return -2;
+
+    // 0. the q columns, column-major, into the last qDim columns of cmQ (consumed
+    //    before the extension writes there)
+    uint64_t *qc = cmq + (uint64_t)(ncols - q_dim) * NExt;
+    if (q_rs == 1) {
+        hipError_t e = hipMemcpy2DAsync(qc, NExt * 8, q, q_cs * 8, NExt * 8, q_dim, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return (int)e;
+    } else {
+        dim3 tg((unsigned)((NExt + TR_T - 1) / TR_T), (unsigned)((q_dim + TR_T - 1) / TR_T));
+        hipLaunchKernelGGL(k_transpose_rm2cm, tg, dim3(256), 0, st, q, qc, NExt, (uint64_t)q_dim, q_rs, NExt);
+    }
+    // 1. strided inverse DIF over n1 (stride n2*B)
+    if (L1 > 0) {
+        uint32_t lt = NTT_LOG_TILE - L1;
+        if (lt > lb) lt = lb;
+        uint32_t log_tile = L1 + lt;
+        dim3 g((unsigned)(1ull << (lb - lt)), q_dim);
+        hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, qc, NExt, L1,
+                           lb, lt, tb);
+    }
+    // 2a. inverse blocks -> scaled coefficient lines (coef: column c at coef + c*N)
+    hipLaunchKernelGGL(k_q_block_inv, dim3(1u << L1, q_dim), dim3(1u << (lb - 4)), lds_bytes(lb), st, qc, NExt, coef,
+                       N, L1, L2, log_b, q_deg, q_dim, n_inv_ext, shift_in, tb);
+    // 2b. extension blocks (spread, DIT, twiddle) into cmQ
+    hipLaunchKernelGGL(k_lde_block<true>, dim3(1u << L1, ncols), dim3(1u << (lb - 4)), lds_bytes(lb), st, coef, N,
+                       cmq, NExt, L1, L2, log_b, (uint64_t)0, tb);
+    // 3. strided forward DIT over n1
+    if (L1 > 0) {
+        uint32_t lt = NTT_LOG_TILE - L1;
+        if (lt > lb) lt = lb;
+        uint32_t log_tile = L1 + lt;
+        dim3 g((unsigned)(1ull << (lb - lt)), ncols);
+        hipLaunchKernelGGL(k_ntt_strided_fwd, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
+                           lb, lt, tb);
+    }
     return (int)hipGetLastError();
 }
 

@@ -293,6 +293,35 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
                        timing_events());
 }
 
+int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_q, uint64_t qRowStride,
+                          uint64_t qColStride, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt, uint64_t qDeg,
+                          uint64_t qDim, uint64_t arity, void *stream)
+{
+    int64_t rc;
+    if (d_nodes) {
+        if ((rc = check_family())) return rc;
+        if ((rc = check_arity(arity))) return rc;
+    }
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (qDeg == 0 || qDim == 0) return fail(-12, "qDeg and qDim must be >= 1");
+    if (qDeg > (1ull << (nBitsExt - nBits))) return fail(-12, "qDeg exceeds the blowup factor NExt / N");
+    if (qDeg * qDim > 65535) return fail(-12, "qDeg * qDim > 65535 is not supported");
+    uint64_t NExt = 1ull << nBitsExt;
+    if (qRowStride == 1 ? (qDim > 1 && qColStride < NExt) : (qColStride != 1 || qRowStride < qDim))
+        return fail(-16, "q must be column-major (rowStride 1, colStride >= NExt) or row-major (colStride 1)");
+    if (!d_cmQ || !d_q || !d_scratch) return fail(-13, "null buffer");
+    NttTablesHost *tb;
+    if ((rc = ensure_tables(&tb))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    int e = ntt_launch_compute_q(d_cmQ, d_q, qRowStride, qRowStride == 1 && qDim == 1 ? NExt : qColStride, d_scratch,
+                                 (uint32_t)nBits, (uint32_t)nBitsExt, (uint32_t)qDeg, (uint32_t)qDim, tb, st);
+    if (e < 0) return fail(-14, "computeQ plan not supported for this shape");
+    if (e) return fail(-100 - e, std::string("computeQ launch: ") + hipGetErrorString((hipError_t)e));
+    if (!d_nodes) return 0;
+    return pil2hip_merkelize(d_nodes, d_cmQ, NExt, qDeg * qDim, 1, NExt, arity, stream);
+}
+
 // ------------------------------------------------------------ batched commit --
 // Many independent commits (uneven shapes) in one hipGraph: every job's
 // LDE + Merkle chain is enqueued on one of S internal streams (longest jobs

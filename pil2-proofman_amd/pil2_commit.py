@@ -48,6 +48,7 @@ _SIGS = {
     "pil2hip_lde": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _vp]),
     "pil2hip_merkelize": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_compute_q": (_i64, [_vp, _vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_batch_release": (None, []),
     "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
@@ -352,6 +353,23 @@ def commit(trace_dev, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, s
     _check(_lib.pil2hip_commit(_ptr(lde), _ptr(nodes), _ptr(trace_dev), _ptr(scratch), arity, n_bits,
                                n_bits_ext, n_cols, _stream(stream)))
     return lde, nodes
+
+
+def compute_q(q_dev, n_bits, n_bits_ext, q_deg, q_dim=3, arity=3, row_major=True, cmq=None, nodes=None,
+              scratch=None, merkelize=True, stream=None):
+    """Starks::computeQ (starks.hpp:187-239) on device: q_dev holds the quotient's
+    NExt x q_dim extended-coset evaluations (row-major, or column-major with
+    row_major=False). Returns (cmQ column-major q_deg*q_dim x NExt, nodes or None)."""
+    n, ne = 1 << n_bits, 1 << n_bits_ext
+    dev = q_dev.device
+    cmq = empty(ne * q_deg * q_dim, dev) if cmq is None else cmq
+    if merkelize and nodes is None:
+        nodes = empty(get_tree_num_elements(ne, arity), dev)
+    scratch = empty(max(n * q_deg * q_dim, 1), dev) if scratch is None else scratch
+    rs, cs = (q_dim, 1) if row_major else (1, ne)
+    _check(_lib.pil2hip_compute_q(_ptr(cmq), _ptr(nodes) if merkelize else None, _ptr(q_dev), rs, cs,
+                                  _ptr(scratch), n_bits, n_bits_ext, q_deg, q_dim, arity, _stream(stream)))
+    return cmq, (nodes if merkelize else None)
 
 
 class Pil2CommitJob(ctypes.Structure):

@@ -100,6 +100,71 @@ def test_lde_matches_compiled_reference_fixtures(oracle, compiled_ref):
 
 
 # -------------------------------------------------------------- Poseidon2 --
+def test_intt_shift_ntt_reference_golden(oracle, refvec):
+    """TEST(GOLDILOCKS_TEST, LDE), test_ntt_cpu.cpp:182-260: INTT(16) of a
+    Fibonacci column, a[i] *= 49^i, zero-extend to 32, NTT(32)."""
+    a = [1, 1]
+    for i in range(2, 16):
+        a.append((a[-1] + a[-2]) % P)
+    c = oracle.intt(np.array(a, dtype=np.uint64))
+    ext = np.zeros(32, dtype=np.uint64)
+    ext[:16] = [(int(v) * pow(49, i, P)) % P for i, v in enumerate(c)]
+    assert [int(v) for v in oracle.ntt(ext)] == hexes(refvec["intt_shift49_ntt_fib16_to_32"])
+
+
+def _ntt_cols(oracle, a, inverse=False):
+    f = oracle.intt if inverse else oracle.ntt
+    return np.stack([f(a[:, c]) for c in range(a.shape[1])], axis=1) if a.size else a
+
+
+def test_ntt_matches_compiled_reference_fixtures(oracle, compiled_ref):
+    """NTT_Goldilocks::NTT / ::INTT (ntt_goldilocks.cpp:211-260) digests."""
+    from golden_helpers import lde_digest
+    for case in compiled_ref["ntt"]:
+        n, c = 1 << case["log_size"], case["n_cols"]
+        a = datagen.uniform(n * c, case["seed"]).reshape(n, c)
+        assert lde_digest(_ntt_cols(oracle, a)) == case["ntt_sha256"], case
+        assert lde_digest(_ntt_cols(oracle, a, inverse=True)) == case["intt_sha256"], case
+
+
+def test_compute_q_matches_compiled_reference_fixtures(oracle, compiled_ref):
+    """Starks::computeQ through the reference's own INTT / NTT."""
+    from golden_helpers import lde_digest
+    for case in compiled_ref["compute_q"]:
+        n, ne = 1 << case["n_bits"], 1 << case["n_bits_ext"]
+        q = datagen.uniform(ne * case["q_dim"], case["seed"]).reshape(ne, case["q_dim"])
+        out = oracle.compute_q(q, n, case["q_deg"])
+        assert [int(v) for v in out[1]] == hexes(case["cmq_row1"]), case
+        assert lde_digest(out) == case["cmq_sha256"], case
+
+
+@pytest.mark.parametrize("lb,lbe,qdeg,qdim", [(3, 5, 4, 3), (4, 6, 2, 3), (5, 6, 2, 1), (2, 4, 3, 3)])
+def test_compute_q_splits_the_quotient(oracle, lb, lbe, qdeg, qdim):
+    """What the mathematics fixes: for q of degree < qDeg*N evaluated on the coset
+    7<w_NExt>, the columns are the evaluations Q_p(x_j) of the unique split
+    q = sum_p x^(p*N) Q_p with deg Q_p < N, so (a) sum_p x_j^(p*N) cmQ[j][p*qDim + d]
+    == q[j][d] at every x_j = 7 w^j and (b) every column interpolates (over the
+    coset) to a polynomial of degree < N."""
+    n, ne = 1 << lb, 1 << lbe
+    coef = datagen.uniform(qdeg * n * qdim, 0xD00 + lb).reshape(qdeg * n, qdim)
+    w = oracle.root(lbe)
+    xs = [(7 * pow(w, j, P)) % P for j in range(ne)]
+    q = np.array([[oracle.poly_eval(coef[:, d], x) for d in range(qdim)] for x in xs], dtype=np.uint64)
+    cm = oracle.compute_q(q, n, qdeg)
+    for j, x in enumerate(xs):
+        xn = pow(x, n, P)
+        for d in range(qdim):
+            acc = sum(int(cm[j, p * qdim + d]) * pow(xn, p, P) for p in range(qdeg)) % P
+            assert acc == int(q[j, d]), (j, d)
+    inv7 = pow(7, P - 2, P)
+    for c in range(qdeg * qdim):   # coset interpolation: coefficient i of Q(7x) is a_i 7^i
+        a = oracle.intt(cm[:, c])
+        assert not a[n:].any()
+        p, d = divmod(c, qdim)
+        want = [(int(coef[p * n + i, d])) for i in range(n)]
+        assert [(int(a[i]) * pow(inv7, i, P)) % P for i in range(n)] == want
+
+
 @pytest.mark.parametrize("w", [8, 12, 16])
 def test_permute_reference_goldens(oracle, refvec, w):
     st = oracle.permute(np.arange(w, dtype=np.uint64))
