@@ -35,6 +35,8 @@ CONFIGS = {
     "batched": dict(n_instances=64, arity=3, seed=0x5EED0003),
     # cubic-extension polynomial on a 2^22 domain, fold factor 8 per step down to 2^5
     "fri": dict(n_bits_ext=22, seed=0x5EED0004, arity=3),
+    # quotient commit: q of 2^24 rows x 3 (cubic) -> qDeg 4 pieces of 2^22, extended to 2^24, merkelized
+    "compute_q": dict(n_bits=22, n_bits_ext=24, q_deg=4, q_dim=3, arity=3, seed=0x5EED0006),
     # Poisson stream of 2000 jobs (log2 rows U{12..22}), HBM budget 200 GB, 32 openings per job
     "serving": dict(n_jobs=2000, arity=3, seed=0x5EED0005, budget=200e9, queries=32, llv=2, load=1.2),
 }
@@ -477,6 +479,100 @@ def run_serving(args, cfg):
         sys.exit(1)
 
 
+def run_compute_q(args, cfg):
+    """Quotient commit: Starks::computeQ (starks.hpp:187-239) + the stage's
+    merkelize. q (row-major NExt x qDim, the prover's layout) is resident in HBM;
+    one step = INTT over NExt, regrouping by shiftIn^p, zero-extended NTT over NExt
+    of qDeg*qDim columns, Merkle tree. value = NExt * qDeg*qDim / s (the columns
+    committed, as for the trace commit)."""
+    import numpy as np
+    import torch
+    import datagen
+    import pil2_commit as pc
+
+    nb, nbe, qdeg, qdim, arity = cfg["n_bits"], cfg["n_bits_ext"], cfg["q_deg"], cfg["q_dim"], cfg["arity"]
+    n, ne, width, nc = 1 << nb, 1 << nbe, 4 * arity, qdeg * qdim
+    pc.set_hash_family(2)
+    dev = torch.device("cuda:0")
+    q = pc.empty(ne * qdim, dev)
+    pc.fill_uniform(q, cfg["seed"])
+    cmq = pc.empty(ne * nc, dev)
+    nodes = pc.empty(pc.get_tree_num_elements(ne, arity), dev)
+    scratch = pc.empty(n * nc, dev)
+    step = lambda: pc.compute_q(q, nb, nbe, qdeg, q_dim=qdim, arity=arity, cmq=cmq, nodes=nodes, scratch=scratch)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # phases on the same stream: the transform alone, then the tree alone
+    st = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(st)
+    pc.compute_q(q, nb, nbe, qdeg, q_dim=qdim, arity=arity, cmq=cmq, scratch=scratch, merkelize=False)
+    e1.record(st)
+    pc._check(pc.lib.pil2hip_merkelize(pc._ptr(nodes), pc._ptr(cmq), ne, nc, 1, ne, arity, pc._stream()))
+    e2.record(st)
+    torch.cuda.synchronize()
+    t_ms, m_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    root = pc.to_host(nodes)[-4:]
+    rate = width - 4
+    perms = ne * ((nc + rate - 1) // rate)
+    # transform: algorithmic bytes = read q once, write cmQ once
+    q_bytes = 8 * ne * qdim + 8 * ne * nc
+    out = {
+        "metric": METRIC, "value": round(ne * nc / dt / 1e9, 4), "unit": "GElem/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks field, exact integer)",
+        "data": "synthetic: seeded uniform Goldilocks elements generated in HBM",
+        "config": {"workload": "compute_q: q 2^%d x %d (cubic) -> %d pieces of 2^%d extended to 2^%d, Poseidon2 W%d "
+                               "arity %d Merkle commit" % (nbe, qdim, qdeg, nb, nbe, width, arity),
+                   "n_bits": nb, "n_bits_ext": nbe, "q_deg": qdeg, "q_dim": qdim, "arity": arity,
+                   "seed": cfg["seed"]},
+        "phases_ms": {"transform": round(t_ms, 4), "merkelize": round(m_ms, 4)},
+        "roofline": _leaf_roofline(width, perms, m_ms, 8 * ne * nc + 32 * ne, None, 1),
+        "roofline_transform": {"bound": "hbm", "achieved": q_bytes / (t_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": q_bytes / (t_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                               "traffic": None, "avg_ms": t_ms},
+        "root": ["%016x" % v for v in root],
+        "device": torch.cuda.get_device_name(0), "host": platform.node(),
+    }
+    out["roofline"]["note"] = "merkelize phase (leaves + levels) timed as one"
+    if not args.no_cpu_baseline:
+        from oracle.oracle import Oracle, RefLib
+        o = Oracle()
+        try:
+            lib, kind, threads = RefLib(), "reference", None
+            threads = lib.num_threads()
+        except (OSError, FileNotFoundError):
+            lib, kind, threads = o, "port", o.num_threads()
+        def cpu_run(sb, sbe):
+            sn, sne = 1 << sb, 1 << sbe
+            hq = datagen.uniform(sne * qdim, cfg["seed"]).reshape(sne, qdim)
+            t0 = time.perf_counter()
+            cm = lib.compute_q(hq, sn, qdeg)
+            if kind == "reference":
+                lib.merkletree(cm, arity, o.tree_num_elements(sne, arity))
+            else:
+                o.merkletree(cm, arity)
+            return time.perf_counter() - t0
+
+        # a quarter of the rows first; the whole commit when that took under a fifth of the budget
+        sb, sbe = nb - 2, nbe - 2
+        cdt = cpu_run(sb, sbe)
+        if cdt < args.cpu_seconds / 5:
+            sb, sbe = nb, nbe
+            cdt = cpu_run(sb, sbe)
+        sne = 1 << sbe
+        out["cpu_baseline"] = {"value": sne * nc / cdt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+                               "sample": "computeQ 2^%d x %d -> %d x 2^%d extended to 2^%d + W%d Merkle; %.2f s"
+                                         % (sbe, qdim, qdeg, sb, sbe, width, cdt), **host_info()}
+    print(json.dumps(out), flush=True)
+
+
 def _profile_traffic(kernel, config):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
@@ -539,6 +635,9 @@ def main():
         return
     if args.config == "fri":
         run_fri(args, cfg)
+        return
+    if args.config == "compute_q":
+        run_compute_q(args, cfg)
         return
     nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
     n, ne, width = 1 << nb, 1 << nbe, 4 * arity

@@ -57,6 +57,39 @@ void *gen_device_buffers(uint32_t node_rank, uint32_t node_size, const int32_t *
                          uint32_t max_n_bits_ext);
 void free_device_buffers(void *d_buffers);
 
+/* ConstTree (pil2-stark/src/starkpil/const_pols.hpp:25-103), GL trees; the
+ * reference entry points (starks_api.hpp:56-63) take a StarkInfo, these take its
+ * four numbers. The const tree buffer is the LDE of the constant polynomials,
+ * row-major 2^nBitsExt x nConstants, followed by the Merkle nodes (the CPU
+ * layout; the same bytes are the file).
+ * getConstTreeSizeGL (const_pols.hpp:25-30): words in the buffer. */
+uint64_t pil2hip_const_tree_size(uint64_t nBitsExt, uint64_t nConstants, uint64_t arity);
+
+/* calculateConstTreeGL (const_pols.hpp:33-44) on device: d_constPols row-major
+ * 2^nBits x nConstants; d_tree receives pil2hip_const_tree_size words;
+ * d_scratch >= (2^nBits + 2^nBitsExt) * nConstants words. Asynchronous. */
+int64_t pil2hip_const_tree(uint64_t *d_tree, const uint64_t *d_constPols, uint64_t *d_scratch, uint64_t nBits,
+                           uint64_t nBitsExt, uint64_t nConstants, uint64_t arity, void *stream);
+
+/* calculate_const_tree (starks_api.hpp:61) with host buffers: pConstPols row-major
+ * 2^nBits x nConstants in, pConstTree (pil2hip_const_tree_size words) out.
+ * d_buffers (gen_device_buffers) selects the stream, or NULL. Synchronous. */
+int64_t pil2hip_calculate_const_tree(void *pConstPols, void *pConstTree, uint64_t nBits, uint64_t nBitsExt,
+                                     uint64_t nConstants, uint64_t arity, void *d_buffers);
+
+/* write_const_tree (starks_api.hpp:63) / MerkleTreeGL::writeFile
+ * (merkleTreeGL.cpp:338-345): writes the buffer as the file. Host only. */
+int64_t pil2hip_write_const_tree(const void *pConstTree, uint64_t nBitsExt, uint64_t nConstants, uint64_t arity,
+                                 const char *treeFilename);
+
+/* load_const_tree (starks_api.hpp:56) / ConstTree::loadConstTree GL branch
+ * (const_pols.hpp:66-103): reads constTreeSize words into pConstTree and, when
+ * verkeyFilename is given, checks the root (last 4 words) against that JSON
+ * array of 4 integers. Returns 1 if loaded (and verified), 0 if not, a negative
+ * code if the key file is not 4 integers. Host only. */
+int64_t pil2hip_load_const_tree(void *pConstTree, const char *treeFilename, uint64_t constTreeSize,
+                                const char *verkeyFilename);
+
 /* starks_api.hpp:203 */
 uint64_t get_num_gpus(void);
 
@@ -86,6 +119,16 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
  * (starks_api.cpp:683-723) without SetupCtx. */
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
+
+/* NTT_Goldilocks::NTT / ::INTT (ntt_goldilocks.hpp:182-191, ntt_goldilocks.cpp:211-260):
+ * per column, out_k = sum_j x_j w^(j*k) with w = W[nBits] (inverse: w^-1 and a
+ * factor 2^-nBits), natural order in and out. rowMajor: element (r, c) at
+ * r*nCols + c (the CPU layout); otherwise column c at c*2^nBits (the GPU layout).
+ * d_dst == d_src or NULL transforms in place. d_scratch: 2 * 2^nBits * nCols words
+ * for row-major data; 2^nBits * nCols for in-place column-major data; unused
+ * otherwise and for 2^nBits < 16. nBits <= 24; nCols == 0 is a no-op. */
+int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch, uint64_t nBits, uint64_t nCols,
+                    bool inverse, bool rowMajor, void *stream);
 
 /* Starks::computeQ (starks.hpp:187-239; GPU twin NTTGoldilocksGPU::computeQ,
  * ntt_goldilocks.cu:1583-1604): the quotient q, qDim columns over the extended

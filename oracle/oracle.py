@@ -259,6 +259,7 @@ class RefLib:
         self.L = L
         L.ref_lde.argtypes = [_u64p, _u64p, _u64, _u64, _u64]
         L.ref_ntt.argtypes = [_u64p, _u64, _u64, ctypes.c_int]
+        L.ref_compute_q.argtypes = [_u64p, _u64p, _u64, _u64, _u64, _u64]
         L.ref_merkletree.restype = ctypes.c_int
         L.ref_merkletree.argtypes = [_u64p, _u64p, _u64, _u64, _u64]
         L.ref_permute.restype = ctypes.c_int
@@ -289,6 +290,14 @@ class RefLib:
             a = a.reshape(-1, 1)
         self.L.ref_ntt(_ptr(a), a.shape[0], a.shape[1], int(inverse))
         return a % np.uint64(P)
+
+    def compute_q(self, q, n, q_deg):
+        """q: row-major (NExt, qDim) -> cmQ row-major (NExt, q_deg*qDim)."""
+        q = np.array(q, dtype=np.uint64)
+        n_ext, q_dim = q.shape
+        out = np.zeros((n_ext, q_deg * q_dim), dtype=np.uint64)
+        self.L.ref_compute_q(_ptr(out), _ptr(q), n, n_ext, q_deg, q_dim)
+        return out % np.uint64(P)
 
     def merkletree(self, src, arity, tree_len):
         # The reference's batch leaf/node loops work 8 rows at a time and touch

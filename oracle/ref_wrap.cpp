@@ -8,6 +8,8 @@
 // Each wrapper names the reference entry point it calls:
 //   ref_lde        -> NTT_Goldilocks(N).LDE          ntt_goldilocks.cpp:369-404
 //   ref_ntt        -> NTT_Goldilocks(n).NTT / INTT   ntt_goldilocks.cpp:211-260, ntt_goldilocks.hpp:187-191
+//   ref_compute_q  -> Starks::computeQ's transform steps (starks.hpp:196-226) on
+//                     NTT_Goldilocks::INTT / NTT and Goldilocks::mul
 //   ref_merkletree -> Poseidon2Goldilocks<4a>::merkletree (Auto mode, as
 //                     MerkleTreeGL::merkelize does)  merkleTreeGL.cpp:316-336
 //   ref_permute    -> Poseidon2Goldilocks<W>::permute poseidon2_goldilocks.hpp:252-272
@@ -19,6 +21,7 @@
 #include "ntt_goldilocks.hpp"
 #include "poseidon2_goldilocks.hpp"
 #include <omp.h>
+#include <vector>
 
 template <uint32_t W>
 static void perm(uint64_t *st)
@@ -46,6 +49,27 @@ void ref_ntt(uint64_t *a, uint64_t size, uint64_t ncols, int inverse)
         ntt.INTT(x, x, size, ncols);
     else
         ntt.NTT(x, x, size, ncols);
+}
+
+// q: row-major NExt x qDim (transformed in place, as in the reference);
+// cmq: row-major NExt x qDeg*qDim
+void ref_compute_q(uint64_t *cmq, uint64_t *q, uint64_t N, uint64_t NExt, uint64_t qDeg, uint64_t qDim)
+{
+    NTT_Goldilocks ntt(NExt);
+    Goldilocks::Element *qe = (Goldilocks::Element *)q, *cm = (Goldilocks::Element *)cmq;
+    ntt.INTT(qe, qe, NExt, qDim);
+    Goldilocks::Element shiftIn = Goldilocks::exp(Goldilocks::inv(Goldilocks::shift()), N);
+    std::vector<Goldilocks::Element> S(qDeg);
+    S[0] = Goldilocks::one();
+    for (uint64_t p = 1; p < qDeg; p++) S[p] = Goldilocks::mul(S[p - 1], shiftIn);
+    uint64_t nc = qDeg * qDim;
+#pragma omp parallel for collapse(2)
+    for (uint64_t p = 0; p < qDeg; p++)
+        for (uint64_t i = 0; i < N; i++)
+            for (uint64_t d = 0; d < qDim; d++) cm[i * nc + p * qDim + d] = Goldilocks::mul(qe[(p * N + i) * qDim + d], S[p]);
+#pragma omp parallel for
+    for (uint64_t k = N * nc; k < NExt * nc; k++) cm[k] = Goldilocks::zero();
+    ntt.NTT(cm, cm, NExt, nc);
 }
 
 int ref_merkletree(uint64_t *tree, const uint64_t *src, uint64_t ncols, uint64_t nrows, uint64_t arity)

@@ -23,6 +23,12 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
 extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q_rs, uint64_t q_cs, uint64_t *coef,
                                     uint32_t log_n, uint32_t log_ext, uint32_t q_deg, uint32_t q_dim,
                                     const NttTablesHost *th, hipStream_t st);
+// NTT / INTT (natural order; the inverse includes 1/M): column-major, out of place, M >= 16
+extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, uint64_t ncols, int inverse,
+                              const NttTablesHost *th, hipStream_t st);
+// M < 16, element (r, c) at base[r*rs + c*cs]; dst may equal src
+extern "C" int ntt_launch_ntt_tiny(uint64_t *dst, const uint64_t *src, uint64_t rs, uint64_t cs, uint32_t lm,
+                                   uint64_t ncols, int inverse, hipStream_t st);
 extern "C" int ntt_launch_cm2rm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_cs,
                                 uint64_t out_rs, hipStream_t st);
 extern "C" int ntt_launch_rm2cm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_rs,

@@ -259,8 +259,8 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
 }
 
 // DIT rounds (NTT pass 3): index bits from the bottom; value j times
-// w_{2^(q+r)}^(bitrev(j) * o_lo), then the 2^r-point DIT.
-template <int R>
+// w_{2^(q+r)}^(bitrev(j) * o_lo), then the 2^r-point DIT (INV: inverse roots).
+template <int R, bool INV = false>
 __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint32_t lt, uint32_t q,
                                                   const NttTables &tb)
 {
@@ -276,10 +276,10 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = tb.t14f[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = (INV ? tb.t14i : tb.t14f)[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
             twiddle_rest<R>(v, w);
         }
-        dft_dit<R, false>(v);
+        dft_dit<R, INV>(v);
 #pragma unroll
         for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
     }
@@ -339,6 +339,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
 }
 
 // NTT pass 3: n1-point DIT over m (bit-reversed in, natural out).
+template <bool INV = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
                                                                  uint32_t log_n2, uint32_t lt, NttTables tb)
 {
@@ -352,10 +353,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     while (q < L) {
         uint32_t r = (L - q) >= 4 ? 4 : (L - q);
         switch (r) {
-        case 4: strided_dit_round<16>(s, lt, q, tb); break;
-        case 3: strided_dit_round<8>(s, lt, q, tb); break;
-        case 2: strided_dit_round<4>(s, lt, q, tb); break;
-        default: strided_dit_round<2>(s, lt, q, tb); break;
+        case 4: strided_dit_round<16, INV>(s, lt, q, tb); break;
+        case 3: strided_dit_round<8, INV>(s, lt, q, tb); break;
+        case 2: strided_dit_round<4, INV>(s, lt, q, tb); break;
+        default: strided_dit_round<2, INV>(s, lt, q, tb); break;
         }
         __syncthreads();
         q += r;
@@ -435,7 +436,7 @@ __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t 
 // values. The first round reads coefficient (o_hi << r) | j for every copy
 // o_lo < B (the spread), the last one applies w_NExt^(bitrev_L1(m) * P) and
 // stores position P of block m.
-template <int R>
+template <int R, bool INV = false>
 __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64_t *__restrict__ dst, uint32_t q,
                                                 uint32_t logb, bool first, bool last, uint32_t e1, uint32_t shift_e,
                                                 const NttTables &tb)
@@ -460,17 +461,17 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = tb.t14f[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = (INV ? tb.t14i : tb.t14f)[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
             twiddle_rest<R>(v[gg], w);
         }
-        dft_dit<R, false>(v[gg]);
+        dft_dit<R, INV>(v[gg]);
         if (last) {  // position P of block m times w_NExt^(e1 * P)
             uint64_t lo[R], hi[R];
 #pragma unroll
             for (int j = 0; j < R; j++) {
                 uint32_t e = (e1 * round_m(p, j, q, LR)) << shift_e;
-                lo[j] = tb.twl_f[e & 4095];
-                hi[j] = tb.twh_f[e >> 12];
+                lo[j] = (INV ? tb.twl_i : tb.twl_f)[e & 4095];
+                hi[j] = (INV ? tb.twh_i : tb.twh_f)[e >> 12];
             }
             gl_mul_v<R>(lo, lo, hi);
             gl_mul_v<R>(v[gg], v[gg], lo);
@@ -485,8 +486,9 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
 
 // COEF_IN: the line already holds coefficients in phase 1's output order
 // (position k = coefficient bitrev_{L1+L2}(m*n2 + k), scaled); phase 1 is skipped
-// and the block only extends (computeQ's second half).
-template <bool COEF_IN>
+// and the block only extends (computeQ's second half, and with logb = 0 the
+// block pass of a plain transform). INV: inverse roots in the DIT and twiddles.
+template <bool COEF_IN, bool INV = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
                                                            uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
                                                            uint32_t L2, uint32_t logb, uint64_t n_inv, NttTables tb)
@@ -518,10 +520,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
         uint32_t r = (top - qq) >= 4 ? 4 : (top - qq);
         bool last = (qq + r == top);
         switch (r) {
-        case 4: block_dit_round<16>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        case 3: block_dit_round<8>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        case 2: block_dit_round<4>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        default: block_dit_round<2>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 4: block_dit_round<16, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 3: block_dit_round<8, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 2: block_dit_round<4, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        default: block_dit_round<2, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
         }
         if (!last) __syncthreads();
         first = false;
@@ -617,6 +619,78 @@ __global__ __launch_bounds__(256) void k_q_tiny(const uint64_t *__restrict__ q, 
     out[(uint64_t)c * ne + j] = acc;
 }
 
+// ------------------------------------------------------- plain NTT / INTT ----
+// NTT_Goldilocks::NTT / INTT (ntt_goldilocks.cpp:211-260): natural order in and
+// out, the inverse scaled by 1/M. Plan: a bit-reversal pass (which also applies
+// 1/M for the inverse) into the order k_lde_block<true> expects with logb = 0,
+// then that block pass and the strided DIT pass, both with the transform's roots.
+//
+// Bit reversal, M = 2^lm >= 2^12: position pos = (h, mid, l) with h the top 6 bits
+// and l the low 6 bits takes source index bitrev(pos) = (bitrev6(l),
+// bitrev(mid), bitrev6(h)). One block per (mid, column) moves a 64 x 64 tile:
+// reads 64 runs of 64 words (rows a = bitrev6(l)), writes 64 runs of 64 words.
+#define BR_T 64
+__global__ __launch_bounds__(256) void k_bitrev_tile(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                                     uint64_t cs, uint32_t lm, uint64_t scale)
+{
+    __shared__ uint64_t tile[BR_T * (BR_T + 1)];
+    uint32_t mid = blockIdx.x;
+    uint64_t hi_stride = 1ull << (lm - 6);
+    uint64_t src_mid = (uint64_t)bitrev32(mid, lm - 12) << 6, dst_mid = (uint64_t)mid << 6;
+    const uint64_t *s = src + (uint64_t)blockIdx.y * cs;
+    uint64_t *d = dst + (uint64_t)blockIdx.y * cs;
+    for (uint32_t x = threadIdx.x; x < BR_T * BR_T; x += blockDim.x) {
+        uint32_t a = x >> 6, b = x & 63;
+        tile[a * (BR_T + 1) + b] = gl_canon(s[a * hi_stride + src_mid + b]);
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < BR_T * BR_T; x += blockDim.x) {
+        uint32_t h = x >> 6, l = x & 63;
+        uint64_t v = tile[bitrev32(l, 6) * (BR_T + 1) + bitrev32(h, 6)];
+        d[h * hi_stride + dst_mid + l] = scale == 1 ? v : gl_mul(v, scale);
+    }
+}
+
+// M < 2^12: one block per column holds the whole column in LDS.
+__global__ __launch_bounds__(256) void k_bitrev_small(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                                      uint64_t cs, uint32_t lm, uint64_t scale)
+{
+    __shared__ uint64_t col[1u << 11];
+    const uint64_t *s = src + (uint64_t)blockIdx.x * cs;
+    uint64_t *d = dst + (uint64_t)blockIdx.x * cs;
+    uint32_t m = 1u << lm;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) col[i] = gl_canon(s[i]);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        uint64_t v = col[bitrev32(i, lm)];
+        d[i] = scale == 1 ? v : gl_mul(v, scale);
+    }
+}
+
+// M < 16: one lane per column, out_k = scale * sum_j x_j w^(j*k), element (r, c)
+// at base[r*rs + c*cs]; the column is read completely before it is written, so
+// dst may equal src.
+__global__ __launch_bounds__(256) void k_ntt_tiny(const uint64_t *src, uint64_t *dst, uint64_t rs, uint64_t cs,
+                                                  uint64_t ncols, uint32_t lm, uint64_t w, uint64_t scale)
+{
+    uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    uint32_t m = 1u << lm;
+    uint64_t x[8], y[8];
+    for (uint32_t j = 0; j < m; j++) x[j] = gl_canon(src[j * rs + c * cs]);
+    uint64_t wk = 1;
+    for (uint32_t k = 0; k < m; k++) {
+        uint64_t acc = 0, p = 1;
+        for (uint32_t j = 0; j < m; j++) {
+            acc = gl_add(acc, gl_mul(x[j], p));
+            p = gl_mul(p, wk);
+        }
+        y[k] = gl_mul(acc, scale);
+        wk = gl_mul(wk, w);
+    }
+    for (uint32_t k = 0; k < m; k++) dst[k * rs + c * cs] = y[k];
+}
+
 // ------------------------------------------------------------ launchers ----
 static const uint64_t kW[33] = {
     0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
@@ -650,7 +724,11 @@ extern "C" int ntt_init_device(void)
         int bytes = (int)(NTT_LDS_WORDS * 8);
         e = hipFuncSetAttribute((const void *)k_ntt_strided_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_lde_block<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void *)k_lde_block<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
@@ -720,7 +798,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         if (lt > l2b) lt = l2b;
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
-        hipLaunchKernelGGL(k_ntt_strided_fwd, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
+        hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
                            N << log_b, L1, l2b, lt, tb);
     }
     if (ev) (void)hipEventRecord(ev[3], st);
@@ -783,8 +861,60 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         if (lt > lb) lt = lb;
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), ncols);
-        hipLaunchKernelGGL(k_ntt_strided_fwd, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
+        hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
                            lb, lt, tb);
+    }
+    return (int)hipGetLastError();
+}
+
+// M < 16, any layout (element (r, c) at base[r*rs + c*cs]); dst may equal src
+extern "C" int ntt_launch_ntt_tiny(uint64_t *dst, const uint64_t *src, uint64_t rs, uint64_t cs, uint32_t lm,
+                                   uint64_t ncols, int inverse, hipStream_t st)
+{
+    if (lm >= 4) return -3;
+    uint64_t w = inverse ? gl_inv(kW[lm]) : kW[lm];
+    uint64_t scale = inverse ? gl_inv(1ull << lm) : 1;
+    hipLaunchKernelGGL(k_ntt_tiny, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, st, src, dst, rs, cs, ncols,
+                       lm, w, scale);
+    return (int)hipGetLastError();
+}
+
+// M >= 16, column-major (column c at base + c*M), dst != src
+extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, uint64_t ncols, int inverse,
+                              const NttTablesHost *th, hipStream_t st)
+{
+    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    if (lm > NTT_MAX_LOG) return -3;
+    uint64_t M = 1ull << lm;
+    uint64_t scale = inverse ? gl_inv(M % GL_P) : 1;
+    if (lm < 4 || src == dst) return -3;  // column-major, out of place, M >= 16
+    // 1. bit reversal (x 1/M for the inverse)
+    if (lm >= 12)
+        hipLaunchKernelGGL(k_bitrev_tile, dim3(1u << (lm - 12), (unsigned)ncols), dim3(256), 0, st, src, dst, M, lm,
+                           scale);
+    else
+        hipLaunchKernelGGL(k_bitrev_small, dim3((unsigned)ncols), dim3(256), 0, st, src, dst, M, lm, scale);
+    // 2. block DIT over the low L2 bits, then 3. strided DIT over n1
+    uint32_t L2 = lm < NTT_LOG_TILE ? lm : NTT_LOG_TILE, L1 = lm - L2;
+    if (L1 > 10) return -2;
+    dim3 g(1u << L1, (unsigned)ncols);
+    if (inverse)
+        hipLaunchKernelGGL((k_lde_block<true, true>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
+                           L2, 0u, (uint64_t)0, tb);
+    else
+        hipLaunchKernelGGL((k_lde_block<true, false>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
+                           L2, 0u, (uint64_t)0, tb);
+    if (L1 > 0) {
+        uint32_t lt = NTT_LOG_TILE - L1;
+        if (lt > L2) lt = L2;
+        uint32_t log_tile = L1 + lt;
+        dim3 gs((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
+        if (inverse)
+            hipLaunchKernelGGL(k_ntt_strided_fwd<true>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst, M,
+                               L1, L2, lt, tb);
+        else
+            hipLaunchKernelGGL(k_ntt_strided_fwd<false>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst,
+                               M, L1, L2, lt, tb);
     }
     return (int)hipGetLastError();
 }

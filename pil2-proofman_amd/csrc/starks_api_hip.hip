@@ -293,6 +293,43 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
                        timing_events());
 }
 
+int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch, uint64_t nBits, uint64_t nCols,
+                    bool inverse, bool rowMajor, void *stream)
+{
+    if (nBits > NTT_MAX_LOG) return fail(-11, "size > 2^24 is not supported");
+    if (nCols == 0) return 0;
+    if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
+    if (!d_src) return fail(-13, "null buffer");
+    if (!d_dst) d_dst = (uint64_t *)d_src;  // NTT_Goldilocks: dst == NULL transforms in place
+    uint64_t M = 1ull << nBits;
+    hipStream_t st = (hipStream_t)stream;
+    int e;
+    if (nBits < 4) {
+        e = ntt_launch_ntt_tiny(d_dst, d_src, rowMajor ? nCols : 1, rowMajor ? 1 : M, (uint32_t)nBits, nCols, inverse,
+                                st);
+    } else {
+        NttTablesHost *tb;
+        int64_t rc = ensure_tables(&tb);
+        if (rc) return rc;
+        if (rowMajor) {  // rows -> columns, transform out of place, columns -> rows
+            if (!d_scratch) return fail(-13, "row-major transforms need a scratch of 2 * 2^nBits * nCols words");
+            uint64_t *a = d_scratch, *b = d_scratch + M * nCols;
+            e = ntt_launch_rm2cm(a, d_src, M, nCols, nCols, M, st);
+            if (!e) e = ntt_launch_ntt(b, a, (uint32_t)nBits, nCols, inverse, tb, st);
+            if (!e) e = ntt_launch_cm2rm(d_dst, b, M, nCols, M, nCols, st);
+        } else if (d_dst == d_src) {
+            if (!d_scratch) return fail(-13, "in-place column-major transforms need a scratch of 2^nBits * nCols words");
+            e = ntt_launch_ntt(d_scratch, d_src, (uint32_t)nBits, nCols, inverse, tb, st);
+            if (!e) e = (int)hipMemcpyAsync(d_dst, d_scratch, M * nCols * 8, hipMemcpyDeviceToDevice, st);
+        } else {
+            e = ntt_launch_ntt(d_dst, d_src, (uint32_t)nBits, nCols, inverse, tb, st);
+        }
+    }
+    if (e < 0) return fail(-14, "NTT plan not supported for this shape");
+    if (e) return fail(-100 - e, std::string("NTT launch: ") + hipGetErrorString((hipError_t)e));
+    return 0;
+}
+
 int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_q, uint64_t qRowStride,
                           uint64_t qColStride, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt, uint64_t qDeg,
                           uint64_t qDim, uint64_t arity, void *stream)
@@ -858,6 +895,114 @@ void write_custom_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nB
     t_err.clear();
     if (write_custom_commit_impl(root, arity, nBits, nBitsExt, nCols, d_buffers, buffer, bufferFile))
         legacy_error("write_custom_commit");
+}
+
+// ---------------------------------------------------------------- const tree --
+// The GL const tree buffer (const_pols.hpp:25-44): the LDE of the constant
+// polynomials, row-major NExt x nConstants, followed by the Merkle nodes.
+uint64_t pil2hip_const_tree_size(uint64_t nBitsExt, uint64_t nConstants, uint64_t arity)
+{
+    uint64_t NExt = 1ull << nBitsExt;
+    return NExt * nConstants + pil2hip_tree_num_elements(NExt, arity);
+}
+
+int64_t pil2hip_const_tree(uint64_t *d_tree, const uint64_t *d_constPols, uint64_t *d_scratch, uint64_t nBits,
+                           uint64_t nBitsExt, uint64_t nConstants, uint64_t arity, void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (!d_tree || (nConstants && (!d_constPols || !d_scratch))) return fail(-13, "null buffer");
+    uint64_t NExt = 1ull << nBitsExt, C = nConstants;
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t *lde_cm = d_scratch, *tmp = d_scratch + NExt * C;
+    if (C && (rc = lde_impl(lde_cm, d_constPols, tmp, nBits, nBitsExt, C, st, nullptr))) return rc;
+    if ((rc = pil2hip_merkelize(d_tree + NExt * C, lde_cm, NExt, C, 1, NExt, arity, stream))) return rc;
+    if (C && (rc = pil2hip_transpose_cm2rm(d_tree, lde_cm, NExt, C, NExt, C, stream))) return rc;
+    return 0;
+}
+
+int64_t pil2hip_calculate_const_tree(void *pConstPols, void *pConstTree, uint64_t nBits, uint64_t nBitsExt,
+                                     uint64_t nConstants, uint64_t arity, void *d_buffers)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt < nBits || nBitsExt > NTT_MAX_LOG) return fail(-11, "unsupported nBits / nBitsExt");
+    if (!pConstTree || (nConstants && !pConstPols)) return fail(-13, "null buffer");
+    uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt, C = nConstants;
+    uint64_t tsz = pil2hip_const_tree_size(nBitsExt, C, arity), ssz = (N + NExt) * C;
+    hipStream_t st = d_buffers ? ((DeviceBuffersHip *)d_buffers)->stream : (hipStream_t)0;
+    uint64_t *d_pols = nullptr, *d_tree = nullptr, *d_scratch = nullptr;
+    auto cleanup = [&]() { (void)hipFree(d_pols); (void)hipFree(d_tree); (void)hipFree(d_scratch); };
+    if (hipMalloc(&d_pols, (N * C ? N * C : 1) * 8) != hipSuccess || hipMalloc(&d_tree, tsz * 8) != hipSuccess ||
+        hipMalloc(&d_scratch, (ssz ? ssz : 1) * 8) != hipSuccess) {
+        cleanup();
+        return fail(-22, "calculate_const_tree: device allocation failed");
+    }
+    if (N * C && hipMemcpyAsync(d_pols, pConstPols, N * C * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = fail(-23, "calculate_const_tree: H2D copy failed");
+    if (!rc) rc = pil2hip_const_tree(d_tree, d_pols, d_scratch, nBits, nBitsExt, C, arity, st);
+    if (!rc && hipMemcpyAsync(pConstTree, d_tree, tsz * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = fail(-24, "calculate_const_tree: D2H copy failed");
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(-25, "stream sync failed");
+    cleanup();
+    return rc;
+}
+
+// MerkleTreeGL::writeFile (merkleTreeGL.cpp:338-345): the source then the nodes,
+// i.e. the whole const tree buffer in order.
+int64_t pil2hip_write_const_tree(const void *pConstTree, uint64_t nBitsExt, uint64_t nConstants, uint64_t arity,
+                                 const char *treeFilename)
+{
+    if (!pConstTree || !treeFilename) return fail(-13, "null buffer");
+    uint64_t n = pil2hip_const_tree_size(nBitsExt, nConstants, arity);
+    FILE *f = fopen(treeFilename, "wb");
+    if (!f) return fail(-20, std::string("cannot open ") + treeFilename);
+    bool ok = fwrite(pConstTree, 8, n, f) == n;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : fail(-21, std::string("short write to ") + treeFilename);
+}
+
+// ConstTree::loadConstTree (const_pols.hpp:66-103), GL branch: read constTreeSize
+// words, then compare the root (the last 4 words) with the verification key, a
+// JSON array of 4 integers. 1 = loaded and verified, 0 = not, < 0 = a
+// verification key that is not 4 integers.
+int64_t pil2hip_load_const_tree(void *pConstTree, const char *treeFilename, uint64_t constTreeSize,
+                                const char *verkeyFilename)
+{
+    if (!pConstTree || !treeFilename || constTreeSize < 4) return fail(-13, "null buffer or size below one root");
+    FILE *f = fopen(treeFilename, "rb");
+    if (!f) return 0;
+    bool ok = fread(pConstTree, 8, constTreeSize, f) == constTreeSize;
+    fclose(f);
+    if (!ok || !verkeyFilename || !verkeyFilename[0]) return ok ? 1 : 0;
+    FILE *k = fopen(verkeyFilename, "rb");
+    if (!k) return 0;
+    std::string text;
+    char buf[256];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof(buf), k)) > 0) text.append(buf, got);
+    fclose(k);
+    uint64_t vk[4];
+    int nv = 0;
+    const char *p = text.c_str();
+    while (*p && nv < 4) {
+        if (*p >= '0' && *p <= '9') {
+            char *end;
+            vk[nv++] = strtoull(p, &end, 10);
+            p = end;
+        } else {
+            p++;
+        }
+    }
+    if (nv != 4) return fail(-26, std::string("verification key is not 4 integers: ") + verkeyFilename);
+    const uint64_t *root = (const uint64_t *)pConstTree + constTreeSize - 4;
+    for (int i = 0; i < 4; i++)
+        if (root[i] % GL_P != vk[i]) return 0;
+    return 1;
 }
 
 }  // extern "C"

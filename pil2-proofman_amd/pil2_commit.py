@@ -48,6 +48,12 @@ _SIGS = {
     "pil2hip_lde": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _vp]),
     "pil2hip_merkelize": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_const_tree_size": (_u64, [_u64, _u64, _u64]),
+    "pil2hip_const_tree": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_calculate_const_tree": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_write_const_tree": (_i64, [_vp, _u64, _u64, _u64, ctypes.c_char_p]),
+    "pil2hip_load_const_tree": (_i64, [_vp, ctypes.c_char_p, _u64, ctypes.c_char_p]),
+    "pil2hip_ntt": (_i64, [_vp, _vp, _vp, _u64, _u64, ctypes.c_bool, ctypes.c_bool, _vp]),
     "pil2hip_compute_q": (_i64, [_vp, _vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_batch_release": (None, []),
@@ -158,6 +164,41 @@ def write_custom_commit(root, arity, n_bits, n_bits_ext, n_cols, d_buffers, buff
         raise HipError("write_custom_commit: " + err)
 
 
+def const_tree_size(n_bits_ext, n_constants, arity):
+    """ConstTree::getConstTreeSizeGL (const_pols.hpp:25-30), in words."""
+    return int(_lib.pil2hip_const_tree_size(n_bits_ext, n_constants, arity))
+
+
+def calculate_const_tree(const_pols, n_bits, n_bits_ext, arity, d_buffers=None):
+    """ConstTree::calculateConstTreeGL (const_pols.hpp:33-44): host row-major
+    2^n_bits x nConstants in; the host const tree buffer (row-major LDE, then the
+    Merkle nodes) out."""
+    pols = np.ascontiguousarray(const_pols, dtype=np.uint64)
+    n_const = pols.shape[1] if pols.ndim == 2 else 0
+    tree = np.zeros(const_tree_size(n_bits_ext, n_const, arity), dtype=np.uint64)
+    _check(_lib.pil2hip_calculate_const_tree(pols.ctypes.data_as(_vp) if pols.size else None,
+                                             tree.ctypes.data_as(_vp), n_bits, n_bits_ext, n_const, arity, d_buffers))
+    return tree
+
+
+def write_const_tree(tree, n_bits_ext, n_constants, arity, path):
+    """write_const_tree / MerkleTreeGL::writeFile (merkleTreeGL.cpp:338-345)."""
+    tree = np.ascontiguousarray(tree, dtype=np.uint64)
+    _check(_lib.pil2hip_write_const_tree(tree.ctypes.data_as(_vp), n_bits_ext, n_constants, arity,
+                                         os.fsencode(path)))
+
+
+def load_const_tree(path, size, verkey_path=None):
+    """ConstTree::loadConstTree, GL branch (const_pols.hpp:66-103): returns
+    (loaded-and-verified, buffer)."""
+    tree = np.zeros(int(size), dtype=np.uint64)
+    ok = _lib.pil2hip_load_const_tree(tree.ctypes.data_as(_vp), os.fsencode(path), int(size),
+                                      os.fsencode(verkey_path) if verkey_path else None)
+    if ok < 0:
+        raise HipError("load_const_tree: " + _lib.pil2hip_last_error().decode())
+    return bool(ok), tree
+
+
 def gen_device_buffers(arity=3, max_n_bits_ext=24):
     return _lib.gen_device_buffers(0, 1, None, arity, max_n_bits_ext)
 
@@ -168,7 +209,7 @@ def free_device_buffers(handle):
 
 # ----------------------------------------------------------------- NTT --
 class NTT_Goldilocks:
-    """Mirror of NTT_Goldilocks (ntt_goldilocks.hpp:66-193) for the LDE entry point."""
+    """Mirror of NTT_Goldilocks (ntt_goldilocks.hpp:66-193): NTT, INTT and LDE on device."""
 
     def __init__(self, max_domain_size):
         self.n = int(max_domain_size)
@@ -179,6 +220,26 @@ class NTT_Goldilocks:
         if (1 << lg) != n:
             raise ValueError("size must be a power of two")
         return lg
+
+    def NTT(self, dst, src, size, ncols=1, buffer=None, inverse=False, row_major=True, stream=None):
+        """ntt_goldilocks.cpp:211-260 on device tensors: natural order in and out;
+        row-major size x ncols (the CPU layout) or column-major (row_major=False).
+        dst None (or src) transforms in place. buffer: device scratch, allocated
+        when needed (2*size*ncols words row-major, size*ncols in-place column-major)."""
+        if size == 0 or ncols == 0:
+            return
+        lg = self._log2(size)
+        dst = src if dst is None else dst
+        need = 0 if lg < 4 else (2 * size * ncols if row_major else (size * ncols if dst.data_ptr() == src.data_ptr()
+                                                                       else 0))
+        if need and (buffer is None or buffer.numel() < need):
+            buffer = empty(need, device=src.device)
+        _check(_lib.pil2hip_ntt(_ptr(dst), _ptr(src), _ptr(buffer) if need else None, lg, ncols, bool(inverse),
+                                bool(row_major), _stream(stream)))
+
+    def INTT(self, dst, src, size, ncols=1, buffer=None, row_major=True, stream=None):
+        """ntt_goldilocks.hpp:187-191: the inverse transform (includes 1/size)."""
+        self.NTT(dst, src, size, ncols, buffer, inverse=True, row_major=row_major, stream=stream)
 
     def LDE(self, output, input, n_extended, n, ncols, buffer=None, stream=None):
         """ntt_goldilocks.cpp:369-404. input: device row-major n x ncols; output:

@@ -99,7 +99,6 @@ def test_lde_matches_compiled_reference_fixtures(oracle, compiled_ref):
         assert lde_digest(out) == case["lde_sha256"], case
 
 
-# -------------------------------------------------------------- Poseidon2 --
 def test_intt_shift_ntt_reference_golden(oracle, refvec):
     """TEST(GOLDILOCKS_TEST, LDE), test_ntt_cpu.cpp:182-260: INTT(16) of a
     Fibonacci column, a[i] *= 49^i, zero-extend to 32, NTT(32)."""
@@ -138,6 +137,17 @@ def test_compute_q_matches_compiled_reference_fixtures(oracle, compiled_ref):
         assert lde_digest(out) == case["cmq_sha256"], case
 
 
+def test_compute_q_matches_compiled_reference(oracle):
+    """Starks::computeQ's transform steps on the reference's own classes."""
+    from oracle.oracle import RefLib
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref = RefLib()
+    for lb, lbe, qdeg, qdim in [(0, 1, 2, 3), (3, 5, 4, 3), (12, 14, 3, 3), (15, 16, 2, 3), (9, 9, 1, 2)]:
+        q = datagen.uniform((1 << lbe) * qdim, lb + lbe).reshape(1 << lbe, qdim)
+        assert (ref.compute_q(q, 1 << lb, qdeg) == oracle.compute_q(q, 1 << lb, qdeg)).all(), (lb, lbe)
+
+
 @pytest.mark.parametrize("lb,lbe,qdeg,qdim", [(3, 5, 4, 3), (4, 6, 2, 3), (5, 6, 2, 1), (2, 4, 3, 3)])
 def test_compute_q_splits_the_quotient(oracle, lb, lbe, qdeg, qdim):
     """What the mathematics fixes: for q of degree < qDeg*N evaluated on the coset
@@ -165,6 +175,7 @@ def test_compute_q_splits_the_quotient(oracle, lb, lbe, qdeg, qdim):
         assert [(int(a[i]) * pow(inv7, i, P)) % P for i in range(n)] == want
 
 
+# -------------------------------------------------------------- Poseidon2 --
 @pytest.mark.parametrize("w", [8, 12, 16])
 def test_permute_reference_goldens(oracle, refvec, w):
     st = oracle.permute(np.arange(w, dtype=np.uint64))
