@@ -89,20 +89,24 @@ def cpu_baseline(cfg, target_s=15.0, max_cols=None):
     t_probe = time.perf_counter() - t0
     cols = int(max(1, min(max_cols or cfg["cols"], pc_ * target_s / max(t_probe, 1e-3))))
     trace = datagen.uniform(n * cols, cfg["seed"]).reshape(n, cols)
-    t0 = time.perf_counter()
-    run(trace)
-    dt = time.perf_counter() - t0
-    return {"value": ne * cols / dt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
-            "sample": "2^%d x %d cols -> 2^%d rows (blowup %d), Poseidon2 W%d arity %d; %.2f s"
+    reps, dt = 0, 0.0
+    while reps == 0 or (dt < target_s / 2 and reps < 1000):  # a short workload runs again up to ~target_s/2
+        t0 = time.perf_counter()
+        run(trace)
+        dt += time.perf_counter() - t0
+        reps += 1
+    return {"value": reps * ne * cols / dt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+            "sample": "2^%d x %d cols -> 2^%d rows (blowup %d), Poseidon2 W%d arity %d; %d run(s), %.2f s"
                       % (cfg["n_bits"], cols, cfg["n_bits"] + cfg["blowup_bits"], 1 << cfg["blowup_bits"],
-                         4 * arity, arity, dt),
+                         4 * arity, arity, reps, dt),
             **host_info()}
 
 
-def cpu_baseline_batched(shapes, arity, target_s=15.0):
-    """Reference CPU path (oracle/_ref, all host cores) over the batch's own
-    instances, taken in generator order while the estimated time stays within
-    about `target_s` seconds (rate estimated from the instances already run)."""
+def cpu_baseline_batched(shapes, arity, target_s=15.0, what="batch"):
+    """Reference CPU path (oracle/_ref, all host cores) over the workload's own
+    instances (the batch's, or the served jobs'), taken in generator order while
+    the estimated time stays within about `target_s` seconds (rate estimated from
+    the instances already run)."""
     import datagen
     from oracle.oracle import Oracle, RefLib
     o = Oracle()
@@ -127,8 +131,8 @@ def cpu_baseline_batched(shapes, arity, target_s=15.0):
         elems += ne * c
         used.append("2^%d x %d (x%d)" % (j["n_bits"], c, ne // n))
     return {"value": elems / spent / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
-            "sample": "%d of the batch's instances [%s], LDE + Poseidon2 W%d arity %d Merkle, %.2f s"
-                      % (len(used), ", ".join(used), 4 * arity, arity, spent),
+            "sample": "%d of the %s's instances [%s], LDE + Poseidon2 W%d arity %d Merkle, %.2f s"
+                      % (len(used), what, ", ".join(used), 4 * arity, arity, spent),
             **host_info()}
 
 
@@ -361,21 +365,25 @@ def run_fri(args, cfg):
         t0 = time.perf_counter()
         host = datagen.uniform(3 << nbe, cfg["seed"])
         t_gen = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        ok = True
-        for s_ in range(len(steps)):
-            if s_ > 0:
-                host = o.fri_fold(host, chal[s_], s_, nbe, steps[s_ - 1], steps[s_])
-            if s_ < len(steps) - 1:
-                nxt = steps[s_ + 1]
-                aux = o.fri_transpose(host, 1 << steps[s_], nxt)
-                r = o.merkletree(aux.reshape(1 << nxt, 3 << (steps[s_] - nxt)), arity)[-4:]
-                ok &= ["%016x" % v for v in r] == roots[s_]
-        cpu_dt = time.perf_counter() - t0
+        src_host, ok, cpu_dt, reps = host, True, 0.0, 0
+        while reps == 0 or (cpu_dt < args.cpu_seconds / 2 and reps < 100):
+            host = src_host
+            t0 = time.perf_counter()
+            for s_ in range(len(steps)):
+                if s_ > 0:
+                    host = o.fri_fold(host, chal[s_], s_, nbe, steps[s_ - 1], steps[s_])
+                if s_ < len(steps) - 1:
+                    nxt = steps[s_ + 1]
+                    aux = o.fri_transpose(host, 1 << steps[s_], nxt)
+                    r = o.merkletree(aux.reshape(1 << nxt, 3 << (steps[s_] - nxt)), arity)[-4:]
+                    ok &= ["%016x" % v for v in r] == roots[s_]
+            cpu_dt += time.perf_counter() - t0
+            reps += 1
         ok &= bool((host[:3 << steps[-1]] == final).all())
-        out["cpu_baseline"] = {"value": elems / cpu_dt / 1e9, "unit": out["unit"], "cores": o.num_threads(),
-                               "kind": "port", "sample": "the full layer sequence once (%.2f s; input generation "
-                                                         "%.2f s not counted)" % (cpu_dt, t_gen), **host_info()}
+        out["cpu_baseline"] = {"value": reps * elems / cpu_dt / 1e9, "unit": out["unit"], "cores": o.num_threads(),
+                               "kind": "port", "sample": "the full layer sequence, %d run(s), %.2f s (input "
+                                                         "generation %.2f s not counted)" % (reps, cpu_dt, t_gen),
+                               **host_info()}
         out["parity_vs_oracle"] = ok
     print(json.dumps(out), flush=True)
 
@@ -474,6 +482,9 @@ def run_serving(args, cfg):
         "server": {k2: st[k2] for k2 in ("admitted", "completed", "failed", "peak_bytes", "peak_running")},
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batched(shapes, arity, args.cpu_seconds, what="served job stream")
+        out["cpu_baseline"]["sample"] += " (commits only; the 32 openings per job are not timed on the CPU)"
     print(json.dumps(out), flush=True)
     if bad:
         sys.exit(1)
@@ -518,7 +529,7 @@ def run_compute_q(args, cfg):
     e2.record(st)
     torch.cuda.synchronize()
     t_ms, m_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
-    root = pc.to_host(nodes)[-4:]
+    root = pc.to_host(nodes[-4:])
     rate = width - 4
     perms = ne * ((nc + rate - 1) // rate)
     # transform: algorithmic bytes = read q once, write cmQ once
@@ -533,7 +544,8 @@ def run_compute_q(args, cfg):
                    "n_bits": nb, "n_bits_ext": nbe, "q_deg": qdeg, "q_dim": qdim, "arity": arity,
                    "seed": cfg["seed"]},
         "phases_ms": {"transform": round(t_ms, 4), "merkelize": round(m_ms, 4)},
-        "roofline": _leaf_roofline(width, perms, m_ms, 8 * ne * nc + 32 * ne, None, 1),
+        "roofline": _leaf_roofline(width, perms, m_ms, 8 * ne * nc + 32 * ne,
+                                   _profile_traffic("k_p2_leaves<%d>" % width, "compute_q"), 1),
         "roofline_transform": {"bound": "hbm", "achieved": q_bytes / (t_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": q_bytes / (t_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                                "traffic": None, "avg_ms": t_ms},
@@ -560,16 +572,20 @@ def run_compute_q(args, cfg):
                 o.merkletree(cm, arity)
             return time.perf_counter() - t0
 
-        # a quarter of the rows first; the whole commit when that took under a fifth of the budget
+        # a quarter of the rows first; the whole commit (repeated up to ~cpu_seconds/2) when that was quick
         sb, sbe = nb - 2, nbe - 2
-        cdt = cpu_run(sb, sbe)
+        cdt, reps = cpu_run(sb, sbe), 1
         if cdt < args.cpu_seconds / 5:
             sb, sbe = nb, nbe
-            cdt = cpu_run(sb, sbe)
+            cdt, reps = 0.0, 0
+            while reps == 0 or (cdt < args.cpu_seconds / 2 and reps < 100):
+                cdt += cpu_run(sb, sbe)
+                reps += 1
         sne = 1 << sbe
-        out["cpu_baseline"] = {"value": sne * nc / cdt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
-                               "sample": "computeQ 2^%d x %d -> %d x 2^%d extended to 2^%d + W%d Merkle; %.2f s"
-                                         % (sbe, qdim, qdeg, sb, sbe, width, cdt), **host_info()}
+        out["cpu_baseline"] = {"value": reps * sne * nc / cdt / 1e9, "unit": "GElem/s", "cores": threads,
+                               "kind": kind,
+                               "sample": "computeQ 2^%d x %d -> %d x 2^%d extended to 2^%d + W%d Merkle; %d run(s), "
+                                         "%.2f s" % (sbe, qdim, qdeg, sb, sbe, width, reps, cdt), **host_info()}
     print(json.dumps(out), flush=True)
 
 
