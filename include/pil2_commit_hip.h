@@ -163,12 +163,36 @@ typedef struct Pil2CommitJob {
  * first, each to the least-loaded branch); each job's root is copied to
  * d_roots[4*i .. 4*i+4) (d_roots may be NULL). The instantiated graph is cached
  * per job list (pointers and shapes) and replayed on later calls.
- * n_streams = 0 autotunes: on the first call for a job list, 1, 2, 4, 8 and 16
- * branches are each timed over one replay and the fastest is kept (these runs
- * recompute the same outputs). Returns the branch count used (> 0), 0 for an
- * empty batch, or a negative error code. Asynchronous like pil2hip_commit. */
+ * n_streams = 0 autotunes the branch count on the first call for a job list
+ * (1, 2, 4, 8, 16, each timed over one replay; these runs recompute the same
+ * outputs); n_streams > 0 uses that many. The rest of the plan is the default
+ * (Pil2CommitPlan below; pil2hip_commit_batch_plan takes an explicit one).
+ * Returns the branch count used (> 0), 0 for an empty batch, or a negative error
+ * code. Asynchronous like pil2hip_commit. */
 int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_t n_streams, uint64_t *d_roots,
                              void *stream);
+
+/* Launch plan of a commit. Every plan computes the same values; they differ in
+ * speed only. n_streams = 0 asks the batch autotuner for the branch count; the
+ * other fields left 0 take their defaults.
+ *   n_streams         graph branches, 1..16
+ *   ntt_tile_log      largest NTT block tile, log2 values, 12..14 (a shape that
+ *                     needs a larger tile gets it); smaller tiles fit more blocks per CU
+ *   ntt_radix_log     most index bits per in-tile NTT round: 2, 3, 4 (radix 4, 8, 16)
+ *   coop_max_parents  tree levels with at most this many parents hash with 16 lanes
+ *                     per permutation, larger levels with one lane per parent
+ * The plain entry points use {-, 14, 4, 16384}. */
+typedef struct Pil2CommitPlan {
+    uint32_t n_streams, ntt_tile_log, ntt_radix_log, coop_max_parents;
+} Pil2CommitPlan;
+
+/* pil2hip_commit_batch with an explicit plan: n_streams = 0 is tuned on the first
+ * call for this job list (each branch count timed over one replay), other fields
+ * left 0 take their defaults; plan == NULL = all 0. The plan used is written to
+ * *chosen (may be NULL). Returns the branch count used, 0 for an empty batch, or
+ * a negative code. */
+int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, const Pil2CommitPlan *plan,
+                                  Pil2CommitPlan *chosen, uint64_t *d_roots, void *stream);
 
 /* Destroys every cached batch graph. */
 void pil2hip_batch_release(void);

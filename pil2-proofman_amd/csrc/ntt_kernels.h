@@ -14,9 +14,16 @@ struct NttTablesHost {
     uint64_t *shl, *shh;                   // 4096 entries each
 };
 
+// LDE launch plan: the largest block tile (log2 values, <= 14; a shape that needs a
+// larger one gets it) and the most index bits per in-tile round (radix 2^radix_log).
+struct NttPlan {
+    uint32_t tile_log, radix_log;
+};
+
 extern "C" int ntt_init_device(void);
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
-                              uint64_t ncols, const NttTablesHost *th, hipStream_t st, hipEvent_t *ev);
+                              uint64_t ncols, const NttTablesHost *th, const NttPlan *plan, hipStream_t st,
+                              hipEvent_t *ev);
 // computeQ (INTT over NExt, x shiftIn^p, regroup, zero-extend, NTT over NExt): q element
 // (k, d) at q[k*q_rs + d*q_cs] (q_rs == 1: column-major; else q_cs == 1), cmq column-major
 // (q_deg*q_dim columns of NExt), coef scratch >= q_deg*q_dim*N words.

@@ -296,7 +296,8 @@ __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uin
 // iNTT pass 1: after the DIF the value at m is the transform at bitrev(m); it is
 // stored times w_N^-(i * bitrev_L(m)) (i = i0 + ii, N = n1 * n2).
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt, NttTables tb)
+                                                                 uint32_t log_n2, uint32_t lt, uint32_t rmax,
+                                                                 NttTables tb)
 {
     extern __shared__ uint64_t s[];
     uint64_t n2 = 1ull << log_n2;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     strided_load(s, col, n2, i0, lt, tile);
     int q = (int)L;
     while (q > 0) {
-        int r = q >= 4 ? 4 : q;
+        int r = q >= (int)rmax ? (int)rmax : q;
         q -= r;
         switch (r) {
         case 4: strided_dif_round<16>(s, lt, q, tb); break;
@@ -341,7 +342,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
 // NTT pass 3: n1-point DIT over m (bit-reversed in, natural out).
 template <bool INV = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt, NttTables tb)
+                                                                 uint32_t log_n2, uint32_t lt, uint32_t rmax,
+                                                                 NttTables tb)
 {
     extern __shared__ uint64_t s[];
     uint64_t n2 = 1ull << log_n2;
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     strided_load(s, col, n2, i0, lt, tile);
     uint32_t q = 0;
     while (q < L) {
-        uint32_t r = (L - q) >= 4 ? 4 : (L - q);
+        uint32_t r = (L - q) >= rmax ? rmax : (L - q);
         switch (r) {
         case 4: strided_dit_round<16, INV>(s, lt, q, tb); break;
         case 3: strided_dit_round<8, INV>(s, lt, q, tb); break;
@@ -411,13 +413,14 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
 
 template <int V>
 __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t m, uint32_t L2, uint32_t log_n,
-                                             uint64_t n_inv, const NttTables &tb, bool scale = true)
+                                             uint64_t n_inv, const NttTables &tb, bool scale, uint32_t rmax)
 {
     constexpr int LV = Log2<V>::v;
     bool active = threadIdx.x < ((1u << L2) >> LV);
+    int rm = LV < (int)rmax ? LV : (int)rmax;
     int q = (int)L2;
     while (q > 0) {
-        int r = q >= LV ? LV : q;
+        int r = q >= rm ? rm : q;
         q -= r;
         bool last = (q == 0) && scale;
         if (active) {
@@ -491,7 +494,8 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
 template <bool COEF_IN, bool INV = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
                                                            uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
-                                                           uint32_t L2, uint32_t logb, uint64_t n_inv, NttTables tb)
+                                                           uint32_t L2, uint32_t logb, uint64_t n_inv, uint32_t rmax,
+                                                           NttTables tb)
 {
     extern __shared__ uint64_t s[];
     uint32_t m = blockIdx.x;
@@ -502,10 +506,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
     __syncthreads();
     if constexpr (!COEF_IN) {
         switch (logb) {
-        case 0: block_phase1<16>(s, m, L2, log_n, n_inv, tb); break;
-        case 1: block_phase1<8>(s, m, L2, log_n, n_inv, tb); break;
-        case 2: block_phase1<4>(s, m, L2, log_n, n_inv, tb); break;
-        default: block_phase1<2>(s, m, L2, log_n, n_inv, tb); break;
+        case 0: block_phase1<16>(s, m, L2, log_n, n_inv, tb, true, rmax); break;
+        case 1: block_phase1<8>(s, m, L2, log_n, n_inv, tb, true, rmax); break;
+        case 2: block_phase1<4>(s, m, L2, log_n, n_inv, tb, true, rmax); break;
+        default: block_phase1<2>(s, m, L2, log_n, n_inv, tb, true, rmax); break;
         }
     }
     bool first = true;
@@ -517,7 +521,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
     uint32_t qq = logb;
     first = true;
     while (qq < top) {
-        uint32_t r = (top - qq) >= 4 ? 4 : (top - qq);
+        uint32_t r = (top - qq) >= rmax ? rmax : (top - qq);
         bool last = (qq + r == top);
         switch (r) {
         case 4: block_dit_round<16, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_q_block_inv(const uint64_t *__r
     const uint64_t *src = in + (uint64_t)d * cs_in + ((uint64_t)m << lb);
     for (uint32_t k = threadIdx.x; k < (1u << lb); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
     __syncthreads();
-    block_phase1<16>(s, m, lb, L1 + lb, 0, tb, false);
+    block_phase1<16>(s, m, lb, L1 + lb, 0, tb, false, 4);
     uint32_t n2 = 1u << L2;
     uint64_t sc = n_inv_ext;  // shiftIn^p / NExt
     for (uint32_t p = 0; p < q_deg; p++) {
@@ -743,8 +747,17 @@ extern "C" int ntt_init_device(void)
 
 static size_t lds_bytes(uint32_t log_tile) { return (size_t)8 * ((1u << log_tile) + (1u << log_tile) / 16); }
 
+// Strided-pass tile: n1 = 2^L1 points x 2^lt lines with 2^(L1 + lt) <= 2^tile values and
+// runs of 2^lt >= 16 words (a tile below that runs at the 2^14 maximum instead).
+static uint32_t strided_lt(uint32_t L1, uint32_t lines_log, uint32_t tile)
+{
+    uint32_t lt = tile >= L1 + 4 ? tile - L1 : NTT_LOG_TILE - L1;
+    return lt > lines_log ? lines_log : lt;
+}
+
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
-                              uint64_t ncols, const NttTablesHost *th, hipStream_t st, hipEvent_t *ev)
+                              uint64_t ncols, const NttTablesHost *th, const NttPlan *plan, hipStream_t st,
+                              hipEvent_t *ev)
 {
     // ev (optional): events recorded after passes 0..3 (ev[0..3]) for phase timing
     NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
@@ -767,20 +780,25 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
             if (ev) (void)hipEventRecord(ev[i], st);
         return (int)hipGetLastError();
     }
-    // plan: n2 * B <= 2^14 values per block, n1 = N / n2 <= 2^10
+    // plan: n2 * B <= 2^tile values per block (tile <= 14), n1 = N / n2 <= 2^10; a
+    // shape that needs n1 > 2^10 at the requested tile gets the smallest tile that fits
+    uint32_t tile = plan ? plan->tile_log : NTT_LOG_TILE, rmax = plan ? plan->radix_log : 4;
+    if (tile > NTT_LOG_TILE) tile = NTT_LOG_TILE;
+    if (tile < log_b + 4) tile = log_b + 4;
+    if (rmax < 1 || rmax > 4) rmax = 4;
     uint32_t L2 = log_n;
-    if (L2 + log_b > NTT_LOG_TILE) L2 = NTT_LOG_TILE - log_b;
+    if (L2 + log_b > tile) L2 = tile - log_b;
+    if (log_n - L2 > 10) L2 = log_n - 10;
     uint32_t L1 = log_n - L2;
-    if (L1 > 10 || L2 < 4) return -2;  // n1 <= 2^10 and >= 16 values per block line
+    if (L2 + log_b > NTT_LOG_TILE || L2 < 4) return -2;  // n2*B <= 2^14 and >= 16 values per block line
 
     // 1. strided inverse DIF over n1
     if (L1 > 0) {
-        uint32_t lt = NTT_LOG_TILE - L1;
-        if (lt > L2) lt = L2;
+        uint32_t lt = strided_lt(L1, L2, tile);
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
-                           L2, lt, tb);
+                           L2, lt, rmax, tb);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     // 2. fused block: DIF, coset scale, spread, DIT, twiddle
@@ -788,18 +806,17 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         uint32_t log_tile = L2 + log_b;
         dim3 g((unsigned)(1u << L1), (unsigned)ncols);
         hipLaunchKernelGGL(k_lde_block<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
-                           N << log_b, L1, L2, log_b, n_inv, tb);
+                           N << log_b, L1, L2, log_b, n_inv, rmax, tb);
     }
     if (ev) (void)hipEventRecord(ev[2], st);
     // 3. strided forward DIT over n1
     if (L1 > 0) {
         uint32_t l2b = L2 + log_b;
-        uint32_t lt = NTT_LOG_TILE - L1;
-        if (lt > l2b) lt = l2b;
+        uint32_t lt = strided_lt(L1, l2b, tile);
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
-                           N << log_b, L1, l2b, lt, tb);
+                           N << log_b, L1, l2b, lt, rmax, tb);
     }
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
@@ -847,14 +864,14 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), q_dim);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, qc, NExt, L1,
-                           lb, lt, tb);
+                           lb, lt, 4u, tb);
     }
     // 2a. inverse blocks -> scaled coefficient lines (coef: column c at coef + c*N)
     hipLaunchKernelGGL(k_q_block_inv, dim3(1u << L1, q_dim), dim3(1u << (lb - 4)), lds_bytes(lb), st, qc, NExt, coef,
                        N, L1, L2, log_b, q_deg, q_dim, n_inv_ext, shift_in, tb);
     // 2b. extension blocks (spread, DIT, twiddle) into cmQ
     hipLaunchKernelGGL(k_lde_block<true>, dim3(1u << L1, ncols), dim3(1u << (lb - 4)), lds_bytes(lb), st, coef, N,
-                       cmq, NExt, L1, L2, log_b, (uint64_t)0, tb);
+                       cmq, NExt, L1, L2, log_b, (uint64_t)0, 4u, tb);
     // 3. strided forward DIT over n1
     if (L1 > 0) {
         uint32_t lt = NTT_LOG_TILE - L1;
@@ -862,7 +879,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
-                           lb, lt, tb);
+                           lb, lt, 4u, tb);
     }
     return (int)hipGetLastError();
 }
@@ -900,10 +917,10 @@ extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, u
     dim3 g(1u << L1, (unsigned)ncols);
     if (inverse)
         hipLaunchKernelGGL((k_lde_block<true, true>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
-                           L2, 0u, (uint64_t)0, tb);
+                           L2, 0u, (uint64_t)0, 4u, tb);
     else
         hipLaunchKernelGGL((k_lde_block<true, false>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
-                           L2, 0u, (uint64_t)0, tb);
+                           L2, 0u, (uint64_t)0, 4u, tb);
     if (L1 > 0) {
         uint32_t lt = NTT_LOG_TILE - L1;
         if (lt > L2) lt = L2;
@@ -911,10 +928,10 @@ extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, u
         dim3 gs((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         if (inverse)
             hipLaunchKernelGGL(k_ntt_strided_fwd<true>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst, M,
-                               L1, L2, lt, tb);
+                               L1, L2, lt, 4u, tb);
         else
             hipLaunchKernelGGL(k_ntt_strided_fwd<false>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst,
-                               M, L1, L2, lt, tb);
+                               M, L1, L2, lt, 4u, tb);
     }
     return (int)hipGetLastError();
 }

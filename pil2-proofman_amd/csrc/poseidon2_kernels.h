@@ -4,7 +4,7 @@
 #include <stdint.h>
 
 #define P2_THREADS 256
-// tree levels with at most this many parents use 16 lanes per permutation
+// tree levels with at most this many parents use 16 lanes per permutation (default)
 #ifndef P2_COOP_MAX_PARENTS
 #define P2_COOP_MAX_PARENTS 16384
 #endif
@@ -17,7 +17,8 @@ struct P2TranscriptState {
 
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st);
-extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st);
+extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, uint64_t coop_max,
+                                     hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st);
 extern "C" int p2_launch_transcript(uint32_t width, P2TranscriptState *t, uint64_t *io, uint64_t n, int op,

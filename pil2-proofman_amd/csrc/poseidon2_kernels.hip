@@ -573,14 +573,14 @@ extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs
     return (int)hipGetLastError();
 }
 
-extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, hipStream_t st)
+extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, uint64_t coop_max, hipStream_t st)
 {
     uint64_t arity = width / 4;
     uint64_t pending = nleaves, off = 0;
     while (pending > 1) {
         uint64_t pad = (arity - pending % arity) % arity;
         uint64_t next = (pending + arity - 1) / arity;
-        if (next <= P2_COOP_MAX_PARENTS) {  // latency-bound level: 16 lanes per parent
+        if (next <= coop_max) {  // latency-bound level: 16 lanes per parent
             dim3 g((unsigned)((next * 16 + P2_THREADS - 1) / P2_THREADS));
             switch (width) {
             case 8: hipLaunchKernelGGL(k_p2_nodes_coop<8>, g, dim3(P2_THREADS), 0, st, tree, off, pending, pad, next); break;

@@ -141,8 +141,11 @@ static int64_t check_arity(uint64_t arity)
     return 0;
 }
 
+static const Pil2CommitPlan kDefaultPlan = {1, NTT_LOG_TILE, 4, P2_COOP_MAX_PARENTS};
+
 static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits,
-                        uint64_t nBitsExt, uint64_t nCols, hipStream_t st, hipEvent_t *ev)
+                        uint64_t nBitsExt, uint64_t nCols, hipStream_t st, hipEvent_t *ev,
+                        const Pil2CommitPlan &plan = kDefaultPlan)
 {
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
     if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
@@ -152,7 +155,8 @@ static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_sc
     NttTablesHost *tb;
     int64_t rc = ensure_tables(&tb);
     if (rc) return rc;
-    int e = ntt_launch_lde(d_lde, d_trace, d_scratch, (uint32_t)nBits, (uint32_t)nBitsExt, nCols, tb, st, ev);
+    NttPlan np = {plan.ntt_tile_log, plan.ntt_radix_log};
+    int e = ntt_launch_lde(d_lde, d_trace, d_scratch, (uint32_t)nBits, (uint32_t)nBitsExt, nCols, tb, &np, st, ev);
     if (e < 0) return fail(-14, "LDE plan not supported for this shape");
     if (e) return fail(-100 - e, std::string("LDE launch: ") + hipGetErrorString((hipError_t)e));
     return 0;
@@ -162,7 +166,7 @@ static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_sc
 // g_ev[1..]), the phases are timed and the stream is synchronised.
 static int64_t commit_impl(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch,
                            uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, hipStream_t st,
-                           hipEvent_t *ev)
+                           hipEvent_t *ev, const Pil2CommitPlan &plan = kDefaultPlan)
 {
     int64_t rc = check_family();
     if (rc) return rc;
@@ -170,13 +174,13 @@ static int64_t commit_impl(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d
     if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
     if (!d_nodes) return fail(-13, "null buffer");
     if (ev) (void)hipEventRecord(g_ev[0], st);
-    if ((rc = lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st, ev))) return rc;
+    if ((rc = lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st, ev, plan))) return rc;
     uint64_t NExt = 1ull << nBitsExt;
     uint32_t width = (uint32_t)(4 * arity);
     int e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
     if (e) return fail(-100 - e, "leaf hashing launch failed");
     if (ev) (void)hipEventRecord(g_ev[5], st);
-    e = p2_launch_tree_levels(width, d_nodes, NExt, st);
+    e = p2_launch_tree_levels(width, d_nodes, NExt, plan.coop_max_parents, st);
     if (e) return fail(-100 - e, "tree level launch failed");
     if (ev) {
         HIP_TRY(hipEventRecord(g_ev[6], st), "timing event");
@@ -281,7 +285,7 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
     uint32_t width = (uint32_t)(4 * arity);
     int e = p2_launch_leaves(width, d_src, rowStride, colStride, nCols, nRows, d_nodes, (hipStream_t)stream);
     if (e) return fail(-100 - e, "leaf hashing launch failed");
-    e = p2_launch_tree_levels(width, d_nodes, nRows, (hipStream_t)stream);
+    e = p2_launch_tree_levels(width, d_nodes, nRows, P2_COOP_MAX_PARENTS, (hipStream_t)stream);
     if (e) return fail(-100 - e, "tree level launch failed");
     return 0;
 }
@@ -364,11 +368,13 @@ int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_
 // LDE + Merkle chain is enqueued on one of S internal streams (longest jobs
 // first, each to the least-loaded stream), a one-block kernel copies each root
 // into d_roots, and the whole fork/join capture is instantiated once per job
-// list and replayed. S = 0 lets the autotuner time S in {1, 2, 4, 8, 16} on the
-// first call and keep the fastest for this job list.
+// list and plan and replayed. A branch count of 0 is tuned on the first call
+// (S in {1, 2, 4, 8, 16}, each candidate timed over one replay of the whole
+// batch after one warm replay); the other plan fields left 0 take their
+// defaults (tile 2^14, radix 16, cooperative levels up to 16,384 parents).
 struct BatchGraph {
     std::vector<uint64_t> key;
-    uint32_t streams;
+    Pil2CommitPlan plan;
     hipGraphExec_t exec;
 };
 static std::vector<BatchGraph> g_batch_graphs;
@@ -397,10 +403,11 @@ static int64_t batch_resources()
     return 0;
 }
 
-static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, uint64_t *d_roots,
+static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, const Pil2CommitPlan &plan, uint64_t *d_roots,
                              hipGraphExec_t *out)
 {
     // LPT assignment: jobs by decreasing work, each to the least-loaded stream
+    uint32_t S = plan.n_streams;
     std::vector<uint64_t> order(n);
     for (uint64_t i = 0; i < n; i++) order[i] = i;
     std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return job_work(jobs[a]) > job_work(jobs[b]); });
@@ -425,7 +432,7 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, 
         const Pil2CommitJob &j = jobs[order[k]];
         hipStream_t st = g_batch_streams[assign[order[k]]];
         rc = commit_impl(j.d_lde, j.d_nodes, j.d_trace, j.d_scratch, j.arity, j.n_bits, j.n_bits_ext, j.n_cols, st,
-                         nullptr);
+                         nullptr, plan);
         if (!rc && d_roots) {
             uint64_t nt = pil2hip_tree_num_elements(1ull << j.n_bits_ext, j.arity);
             int e = p2_launch_copy_root(j.d_nodes + nt - 4, d_roots + 4 * order[k], st);
@@ -451,14 +458,45 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, uint32_t S, 
     return 0;
 }
 
-int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_t n_streams, uint64_t *d_roots,
-                             void *stream)
+static bool plan_eq(const Pil2CommitPlan &a, const Pil2CommitPlan &b)
+{
+    return a.n_streams == b.n_streams && a.ntt_tile_log == b.ntt_tile_log && a.ntt_radix_log == b.ntt_radix_log &&
+           a.coop_max_parents == b.coop_max_parents;
+}
+
+// Captures `plan` and times one replay (after a warm one) when `timed`.
+static int64_t batch_try(const Pil2CommitJob *jobs, uint64_t n, const Pil2CommitPlan &plan, uint64_t *d_roots,
+                         hipStream_t st, bool timed, hipEvent_t e0, hipEvent_t e1, hipGraphExec_t *ex, float *ms)
+{
+    int64_t rc = batch_capture(jobs, n, plan, d_roots, ex);
+    if (rc || !timed) return rc;
+    hipError_t te = hipGraphLaunch(*ex, st);
+    if (te == hipSuccess) te = hipEventRecord(e0, st);
+    if (te == hipSuccess) te = hipGraphLaunch(*ex, st);
+    if (te == hipSuccess) te = hipEventRecord(e1, st);
+    if (te == hipSuccess) te = hipEventSynchronize(e1);
+    if (te == hipSuccess) te = hipEventElapsedTime(ms, e0, e1);
+    if (te != hipSuccess) {
+        (void)hipGraphExecDestroy(*ex);
+        *ex = nullptr;
+        return fail(-100 - (int64_t)te, std::string("autotune: ") + hipGetErrorString(te));
+    }
+    return 0;
+}
+
+int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, const Pil2CommitPlan *plan,
+                                  Pil2CommitPlan *chosen, uint64_t *d_roots, void *stream)
 {
     int64_t rc = check_family();
     if (rc) return rc;
     if (n_jobs == 0) return 0;
     if (!jobs) return fail(-13, "null jobs");
-    if (n_streams > kMaxBatchStreams) return fail(-27, "n_streams > 16");
+    Pil2CommitPlan want = plan ? *plan : Pil2CommitPlan{0, 0, 0, 0};
+    if (want.n_streams > kMaxBatchStreams) return fail(-27, "n_streams > 16");
+    if (want.ntt_tile_log && (want.ntt_tile_log < 12 || want.ntt_tile_log > NTT_LOG_TILE))
+        return fail(-27, "ntt_tile_log must be 12..14");
+    if (want.ntt_radix_log && (want.ntt_radix_log < 2 || want.ntt_radix_log > 4))
+        return fail(-27, "ntt_radix_log must be 2..4");
     for (uint64_t i = 0; i < n_jobs; i++) {
         const Pil2CommitJob &j = jobs[i];
         if ((rc = check_arity(j.arity))) return rc;
@@ -478,45 +516,61 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
                                j.n_bits_ext, j.n_cols, j.arity});
     }
     hipStream_t st = (hipStream_t)stream;
+    auto matches = [&](const Pil2CommitPlan &p) {
+        return (!want.n_streams || p.n_streams == want.n_streams) &&
+               (!want.ntt_tile_log || p.ntt_tile_log == want.ntt_tile_log) &&
+               (!want.ntt_radix_log || p.ntt_radix_log == want.ntt_radix_log) &&
+               (!want.coop_max_parents || p.coop_max_parents == want.coop_max_parents);
+    };
     for (auto &g : g_batch_graphs)
-        if (g.key == key && (n_streams == 0 || g.streams == n_streams)) {
+        if (g.key == key && matches(g.plan)) {
             HIP_TRY(hipGraphLaunch(g.exec, st), "graph launch");
-            return g.streams;
+            if (chosen) *chosen = g.plan;
+            return g.plan.n_streams;
         }
-    std::vector<uint32_t> cands;
-    if (n_streams) cands.push_back(n_streams);
-    else
-        for (uint32_t s = 1; s <= kMaxBatchStreams && s <= std::max<uint64_t>(n_jobs, 1); s *= 2) cands.push_back(s);
+    // defaults for the fields left 0; the branch count is then tuned when it was 0
+    Pil2CommitPlan cur = {want.n_streams ? want.n_streams : (uint32_t)std::min<uint64_t>(4, n_jobs),
+                          want.ntt_tile_log ? want.ntt_tile_log : NTT_LOG_TILE,
+                          want.ntt_radix_log ? want.ntt_radix_log : 4u,
+                          want.coop_max_parents ? want.coop_max_parents : (uint32_t)P2_COOP_MAX_PARENTS};
+    std::vector<std::vector<Pil2CommitPlan>> stages;
+    if (!want.n_streams) {
+        std::vector<Pil2CommitPlan> c;
+        for (uint32_t s = 1; s <= kMaxBatchStreams && s <= std::max<uint64_t>(n_jobs, 1); s *= 2)
+            c.push_back({s, 0, 0, 0});
+        stages.push_back(c);
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "event");
     HIP_TRY(hipEventCreate(&e1), "event");
-    float best_ms = 1e30f;
     hipGraphExec_t best = nullptr;
-    uint32_t best_s = 0;
-    for (uint32_t S : cands) {
-        hipGraphExec_t ex = nullptr;
-        if ((rc = batch_capture(jobs, n_jobs, S, d_roots, &ex))) break;
+    float best_ms = 1e30f;
+    Pil2CommitPlan best_plan = cur;
+    bool tuning = !stages.empty();
+    if (!tuning) {
         float ms = 0;
-        if (cands.size() > 1) {  // autotune: one warm launch, one timed launch
-            hipError_t te = hipGraphLaunch(ex, st);
-            if (te == hipSuccess) te = hipEventRecord(e0, st);
-            if (te == hipSuccess) te = hipGraphLaunch(ex, st);
-            if (te == hipSuccess) te = hipEventRecord(e1, st);
-            if (te == hipSuccess) te = hipEventSynchronize(e1);
-            if (te == hipSuccess) te = hipEventElapsedTime(&ms, e0, e1);
-            if (te != hipSuccess) {
+        rc = batch_try(jobs, n_jobs, cur, d_roots, st, false, e0, e1, &best, &ms);
+    }
+    for (size_t si = 0; si < stages.size() && !rc; si++) {
+        Pil2CommitPlan base = best_plan;
+        for (const Pil2CommitPlan &delta : stages[si]) {
+            Pil2CommitPlan cand = base;
+            if (delta.n_streams) cand.n_streams = delta.n_streams;
+            if (delta.ntt_tile_log) cand.ntt_tile_log = delta.ntt_tile_log;
+            if (delta.ntt_radix_log) cand.ntt_radix_log = delta.ntt_radix_log;
+            if (delta.coop_max_parents) cand.coop_max_parents = delta.coop_max_parents;
+            if (best && plan_eq(cand, best_plan)) continue;  // already timed
+            hipGraphExec_t ex = nullptr;
+            float ms = 0;
+            if ((rc = batch_try(jobs, n_jobs, cand, d_roots, st, true, e0, e1, &ex, &ms))) break;
+            if (ms < best_ms) {
+                if (best) (void)hipGraphExecDestroy(best);
+                best = ex;
+                best_ms = ms;
+                best_plan = cand;
+            } else {
                 (void)hipGraphExecDestroy(ex);
-                rc = fail(-100 - (int64_t)te, std::string("autotune: ") + hipGetErrorString(te));
-                break;
             }
-        }
-        if (ms < best_ms) {
-            if (best) (void)hipGraphExecDestroy(best);
-            best = ex;
-            best_ms = ms;
-            best_s = S;
-        } else {
-            (void)hipGraphExecDestroy(ex);
         }
     }
     (void)hipEventDestroy(e0);
@@ -529,9 +583,20 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
         (void)hipGraphExecDestroy(g_batch_graphs.front().exec);
         g_batch_graphs.erase(g_batch_graphs.begin());
     }
-    g_batch_graphs.push_back({key, best_s, best});
+    g_batch_graphs.push_back({key, best_plan, best});
     HIP_TRY(hipGraphLaunch(best, st), "graph launch");
-    return best_s;
+    if (chosen) *chosen = best_plan;
+    return best_plan.n_streams;
+}
+
+int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_t n_streams, uint64_t *d_roots,
+                             void *stream)
+{
+    // n_streams = 0 tunes the whole plan; otherwise the default plan with n_streams branches
+    if (n_streams == 0) return pil2hip_commit_batch_plan(jobs, n_jobs, nullptr, nullptr, d_roots, stream);
+    Pil2CommitPlan p = kDefaultPlan;
+    p.n_streams = n_streams;
+    return pil2hip_commit_batch_plan(jobs, n_jobs, &p, nullptr, d_roots, stream);
 }
 
 void pil2hip_batch_release(void)

@@ -56,6 +56,7 @@ _SIGS = {
     "pil2hip_ntt": (_i64, [_vp, _vp, _vp, _u64, _u64, ctypes.c_bool, ctypes.c_bool, _vp]),
     "pil2hip_compute_q": (_i64, [_vp, _vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
+    "pil2hip_commit_batch_plan": (_i64, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pil2hip_batch_release": (None, []),
     "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
     "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
@@ -439,6 +440,15 @@ class Pil2CommitJob(ctypes.Structure):
                 ("n_bits", _u64), ("n_bits_ext", _u64), ("n_cols", _u64), ("arity", _u64)]
 
 
+class Pil2CommitPlan(ctypes.Structure):
+    """Mirror of the C struct Pil2CommitPlan (include/pil2_commit_hip.h); 0 = tune."""
+    _fields_ = [("n_streams", ctypes.c_uint32), ("ntt_tile_log", ctypes.c_uint32),
+                ("ntt_radix_log", ctypes.c_uint32), ("coop_max_parents", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CommitBatch:
     """Many independent commits launched as one hipGraph (pil2hip_commit_batch).
 
@@ -467,11 +477,22 @@ class CommitBatch:
         dev = jobs[0]["trace"].device if jobs else "cuda"
         self.roots = empty(4 * max(len(jobs), 1), dev)
         self.streams = None
+        self.plan = None
 
-    def launch(self, n_streams=0, stream=None):
-        """Enqueue the batch; returns the number of graph branches used."""
-        rc = _lib.pil2hip_commit_batch(ctypes.cast(self._arr, _vp), len(self.jobs), n_streams,
-                                       _ptr(self.roots), _stream(stream))
+    def launch(self, n_streams=0, stream=None, plan=None):
+        """Enqueue the batch; returns the number of graph branches used. `plan`
+        (a dict of Pil2CommitPlan fields, missing or 0 = tune) selects
+        pil2hip_commit_batch_plan; the plan used is kept in self.plan."""
+        if plan is None:
+            rc = _lib.pil2hip_commit_batch(ctypes.cast(self._arr, _vp), len(self.jobs), n_streams,
+                                           _ptr(self.roots), _stream(stream))
+        else:
+            want = Pil2CommitPlan(**{k: int(v) for k, v in plan.items()})
+            got = Pil2CommitPlan()
+            rc = _lib.pil2hip_commit_batch_plan(ctypes.cast(self._arr, _vp), len(self.jobs), ctypes.byref(want),
+                                                ctypes.byref(got), _ptr(self.roots), _stream(stream))
+            if rc >= 0:
+                self.plan = got.as_dict()
         if rc < 0:
             _check(rc)
         self.streams = int(rc)
