@@ -216,7 +216,8 @@ def run_batched(args, cfg):
     # (b) batched: the first call autotunes the launch plan (graph branches, NTT tile
     # and radix, Merkle level schedule) and instantiates the graph; later calls replay it
     t_tune = time.perf_counter()
-    streams = batch.launch(plan=dict(n_streams=args.batch_streams))
+    streams = batch.launch(plan=dict(parse_plan(args.plan), n_streams=args.batch_streams) if args.plan
+                           else dict(n_streams=args.batch_streams))
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t_tune
     plan = batch.plan
@@ -592,6 +593,48 @@ def run_compute_q(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def parse_plan(text):
+    """'S,tile,radix,coop' -> Pil2CommitPlan fields (0 = default / tuned branch count)."""
+    f = [int(x) for x in text.split(",")]
+    if len(f) != 4:
+        raise SystemExit("--plan takes S,tile,radix,coop")
+    return dict(n_streams=f[0], ntt_tile_log=f[1], ntt_radix_log=f[2], coop_max_parents=f[3])
+
+
+def run_plain_plan(args, cfg):
+    """The plain commit under an explicit launch plan: one job through
+    pil2hip_commit_batch_plan (a one-job graph), timed like the plain path."""
+    import torch
+    import pil2_commit as pc
+
+    nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
+    ne = 1 << nbe
+    pc.set_hash_family(2)
+    dev = torch.device("cuda:0")
+    trace = pc.empty((1 << nb) * cols, dev)
+    pc.fill_uniform(trace, cfg["seed"])
+    plan = dict(parse_plan(args.plan), n_streams=1)
+    b = pc.CommitBatch([{"trace": trace, "n_bits": nb, "n_bits_ext": nbe, "n_cols": cols}], arity=arity)
+    b.launch(plan=plan)
+    for _ in range(args.warmup):
+        b.launch(plan=plan)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch(plan=plan)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {"metric": METRIC, "value": round(ne * cols / dt / 1e9, 4), "unit": "GElem/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks field, exact integer)",
+           "data": "synthetic: seeded uniform Goldilocks elements generated in HBM",
+           "config": {"workload": "%s under launch plan %s (one-job graph)" % (args.config, b.plan),
+                      "plan": b.plan, "n_bits": nb, "n_bits_ext": nbe, "n_cols": cols, "arity": arity},
+           "root": ["%016x" % v for v in b.roots_host()[0]],
+           "device": torch.cuda.get_device_name(0), "host": platform.node()}
+    print(json.dumps(out), flush=True)
+
+
 def _profile_traffic(kernel, config):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
@@ -632,6 +675,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=0, help="serving: number of jobs (default 2000)")
     ap.add_argument("--serve-streams", type=int, default=4, help="serving: device streams")
     ap.add_argument("--serve-inflight", type=int, default=32, help="serving: admitted jobs in flight")
+    ap.add_argument("--plan", default="", metavar="S,TILE,RADIX,COOP",
+                    help="explicit launch plan (plain configs: run as a one-job graph; batched: the batch plan)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -657,6 +702,9 @@ def main():
         return
     if args.config == "compute_q":
         run_compute_q(args, cfg)
+        return
+    if args.plan:
+        run_plain_plan(args, cfg)
         return
     nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
     n, ne, width = 1 << nb, 1 << nbe, 4 * arity

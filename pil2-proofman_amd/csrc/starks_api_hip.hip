@@ -141,7 +141,8 @@ static int64_t check_arity(uint64_t arity)
     return 0;
 }
 
-static const Pil2CommitPlan kDefaultPlan = {1, NTT_LOG_TILE, 4, P2_COOP_MAX_PARENTS};
+// measured on the plain and mixed-batch workloads: 2^13 tiles (two blocks per CU) and radix-8 rounds
+static const Pil2CommitPlan kDefaultPlan = {1, 13, 3, P2_COOP_MAX_PARENTS};
 
 static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits,
                         uint64_t nBitsExt, uint64_t nCols, hipStream_t st, hipEvent_t *ev,
@@ -371,7 +372,7 @@ int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_
 // list and plan and replayed. A branch count of 0 is tuned on the first call
 // (S in {1, 2, 4, 8, 16}, each candidate timed over one replay of the whole
 // batch after one warm replay); the other plan fields left 0 take their
-// defaults (tile 2^14, radix 16, cooperative levels up to 16,384 parents).
+// defaults (tile 2^13, radix 8, cooperative levels up to 16,384 parents).
 struct BatchGraph {
     std::vector<uint64_t> key;
     Pil2CommitPlan plan;
@@ -530,8 +531,8 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
         }
     // defaults for the fields left 0; the branch count is then tuned when it was 0
     Pil2CommitPlan cur = {want.n_streams ? want.n_streams : (uint32_t)std::min<uint64_t>(4, n_jobs),
-                          want.ntt_tile_log ? want.ntt_tile_log : NTT_LOG_TILE,
-                          want.ntt_radix_log ? want.ntt_radix_log : 4u,
+                          want.ntt_tile_log ? want.ntt_tile_log : kDefaultPlan.ntt_tile_log,
+                          want.ntt_radix_log ? want.ntt_radix_log : kDefaultPlan.ntt_radix_log,
                           want.coop_max_parents ? want.coop_max_parents : (uint32_t)P2_COOP_MAX_PARENTS};
     std::vector<std::vector<Pil2CommitPlan>> stages;
     if (!want.n_streams) {
