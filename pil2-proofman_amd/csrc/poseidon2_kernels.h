@@ -9,6 +9,11 @@
 #define P2_COOP_MAX_PARENTS 16384
 #endif
 
+// leaf hashing of at most this many rows uses 16 lanes per row
+#ifndef P2_COOP_LEAF_MAX_ROWS
+#define P2_COOP_LEAF_MAX_ROWS 16384
+#endif
+
 // Transcript sponge state (device memory), see k_transcript.
 struct P2TranscriptState {
     uint64_t state[16], pending[12], out[16];

@@ -388,6 +388,44 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_nodes_coop(uint64_t *__restri
     if (active && i < 4) tree[in_off + (pending + pad) * 4 + par * 4 + i] = gl_canon(s);
 }
 
+// Leaves of short trees, 16 lanes per row (same sponge as k_p2_leaves): lane i < RATE
+// absorbs column c0 + i, capacity lanes RATE..RATE+3 take the previous output words
+// 0..3 (a DPP row rotation by RATE), lanes >= W stay zero.
+template <int W>
+__global__ __launch_bounds__(P2_THREADS) void k_p2_leaves_coop(const uint64_t *__restrict__ src, uint64_t rs,
+                                                               uint64_t cs, uint64_t ncols, uint64_t nrows,
+                                                               uint64_t *__restrict__ out)
+{
+    constexpr int RATE = W - 4;
+    uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    uint32_t i = threadIdx.x & 15;
+    bool active = r < nrows;  // whole rows are active or not
+    if (ncols == 0) {
+        if (active && i < 4) out[r * 4 + i] = 0;
+        return;
+    }
+    const uint64_t *row = src + (active ? r : 0) * rs;
+    uint64_t s = 0;
+    for (uint64_t c0 = 0; c0 < ncols; c0 += RATE) {
+        uint64_t carried = 0;
+        if (c0 > 0) {  // capacity <- previous output words 0..3
+            uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32), z = 0;
+            dpp_ready(lo, hi, z);
+            lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, dpp_row_ror(RATE), 0xF, 0xF, false);
+            hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, dpp_row_ror(RATE), 0xF, 0xF, false);
+            carried = ((uint64_t)hi << 32) | lo;
+        }
+        uint64_t x = 0;
+        if (i < (uint32_t)RATE) {
+            if (active && c0 + i < ncols) x = row[(c0 + i) * cs];
+        } else if (i < (uint32_t)W) {
+            x = carried;
+        }
+        s = p2_permute_coop<W>(x, i);
+    }
+    if (active && i < 4) out[r * 4 + i] = gl_canon(s);
+}
+
 // ----------------------------------------------------------- grinding ----
 // Poseidon2Goldilocks<8>::grinding (poseidon2_goldilocks.cpp:177-232): nonce i
 // is valid when permute([c0, c1, c2, i, 0, 0, 0, 0])[0] < 2^(64 - n_bits). One
@@ -563,6 +601,16 @@ extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs
                                 uint64_t nrows, uint64_t *out, hipStream_t st)
 {
     if (nrows == 0) return 0;
+    if (nrows <= P2_COOP_LEAF_MAX_ROWS) {  // too few rows to fill the GPU: 16 lanes per row
+        dim3 g((unsigned)((nrows * 16 + P2_THREADS - 1) / P2_THREADS));
+        switch (width) {
+        case 8: hipLaunchKernelGGL(k_p2_leaves_coop<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+        case 12: hipLaunchKernelGGL(k_p2_leaves_coop<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+        case 16: hipLaunchKernelGGL(k_p2_leaves_coop<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+        default: return -1;
+        }
+        return (int)hipGetLastError();
+    }
     dim3 g((unsigned)((nrows + P2_THREADS - 1) / P2_THREADS));
     switch (width) {
     case 8: hipLaunchKernelGGL(k_p2_leaves<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
