@@ -206,7 +206,9 @@ __device__ __forceinline__ void twiddle_rest(uint64_t *v, const uint64_t *w)
 
 // Element position of value j of group gidx in a round over index bits [q, q+r)
 // of a tile with 2^lt lines (line index fastest). Returns the tile-linear index
-// (m << lt) | ii and the pieces the kernels need.
+// (m << lt) | ii and the pieces the kernels need. A thread with G groups per round
+// takes gidx = gg * threads + t, so that for each gg consecutive lanes hold
+// consecutive positions (contiguous LDS and global accesses).
 struct RoundPos {
     uint32_t ii, o_lo, o_hi;
 };
@@ -242,7 +244,7 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
     uint32_t t = threadIdx.x;
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(t * G + gg, lt, q);
+        RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
         uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
@@ -269,7 +271,7 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
     uint32_t t = threadIdx.x;
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(t * G + gg, lt, q);
+        RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
         uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
@@ -382,7 +384,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
     uint32_t t = threadIdx.x;
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(t * G + gg, 0, q);
+        RoundPos p = round_pos(gg * ((1u << L2) >> Log2<V>::v) + t, 0, q);
         uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr(round_m(p, j, q, LR))];
@@ -450,7 +452,7 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
     uint64_t v[G][R];
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(t * G + gg, 0, q);
+        RoundPos p = round_pos(gg * blockDim.x + t, 0, q);
 #pragma unroll
         for (int j = 0; j < R; j++) {
             uint32_t P = round_m(p, j, q, LR);
@@ -460,7 +462,7 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
     if (first) __syncthreads();  // the spread overwrites coefficients other threads still read
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(t * G + gg, 0, q);
+        RoundPos p = round_pos(gg * blockDim.x + t, 0, q);
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
