@@ -25,7 +25,7 @@
 // w_{2^(q+r)}^(o*bitrev(j)) (o = the element's index bits below the round).
 // Each thread keeps 16 values per round; rounds exchange through LDS. The first
 // round of a pass reads global memory and the last one writes it, both in runs
-// of >= 128 bytes.
+// of >= 64 bytes (whole 128-byte lines per XCD; see strided_tile).
 #include "gl_field.h"
 #include "ntt_kernels.h"
 
@@ -230,7 +230,7 @@ __device__ __forceinline__ uint32_t round_m(const RoundPos &p, uint32_t j, uint3
 // Tile: n1 = 2^L points (index m, global stride n2) x 2^lt lines (index ii, global
 // offset i0 + ii); column c = blockIdx.y at data + c*cs. The tile holds 2^(L+lt)
 // values, 16 per thread, in LDS at tile index x = (m << lt) | ii. A load phase
-// and a store phase move the tile in runs of 2^lt >= 16 consecutive words; the
+// and a store phase move the tile in runs of 2^lt >= 8 consecutive words; the
 // rounds between them work in LDS only.
 //
 // DIF rounds (iNTT pass 1): index bits from the top, 2^r-point DIF, then value j
@@ -287,6 +287,16 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
     }
 }
 
+// Strided tiles with runs of 2^lt = 8 words use half of each 128-byte line; the
+// tile pair (2u, 2u+1) shares the lines. Workgroups are dispatched round-robin
+// over the 8 XCDs (each with its own L2), so the pair goes to blocks b and b+8,
+// which share an XCD and run close together.
+__device__ __forceinline__ uint32_t strided_tile(uint32_t bx, uint32_t lt)
+{
+    if (lt != 3 || (gridDim.x & 15u)) return bx;
+    return (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u);
+}
+
 __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ col, uint64_t n2,
                                              uint64_t i0, uint32_t lt, uint32_t tile)
 {
@@ -303,7 +313,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
 {
     extern __shared__ uint64_t s[];
     uint64_t n2 = 1ull << log_n2;
-    uint64_t i0 = (uint64_t)blockIdx.x << lt;
+    uint64_t i0 = (uint64_t)strided_tile(blockIdx.x, lt) << lt;
     uint64_t *col = data + (uint64_t)blockIdx.y * cs;
     uint32_t tile = 1u << (L + lt);
     strided_load(s, col, n2, i0, lt, tile);
@@ -349,7 +359,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
 {
     extern __shared__ uint64_t s[];
     uint64_t n2 = 1ull << log_n2;
-    uint64_t i0 = (uint64_t)blockIdx.x << lt;
+    uint64_t i0 = (uint64_t)strided_tile(blockIdx.x, lt) << lt;
     uint64_t *col = data + (uint64_t)blockIdx.y * cs;
     uint32_t tile = 1u << (L + lt);
     strided_load(s, col, n2, i0, lt, tile);
@@ -750,10 +760,12 @@ extern "C" int ntt_init_device(void)
 static size_t lds_bytes(uint32_t log_tile) { return (size_t)8 * ((1u << log_tile) + (1u << log_tile) / 16); }
 
 // Strided-pass tile: n1 = 2^L1 points x 2^lt lines with 2^(L1 + lt) <= 2^tile values and
-// runs of 2^lt >= 16 words (a tile below that runs at the 2^14 maximum instead).
+// runs of 2^lt >= 8 words (a tile below that runs at the 2^14 maximum instead). Runs of
+// 8 words (half lines, tile pairs on one XCD) let a 2^13 tile hold n1 = 2^10 points, so
+// two blocks share a CU and one block's loads overlap the other's rounds.
 static uint32_t strided_lt(uint32_t L1, uint32_t lines_log, uint32_t tile)
 {
-    uint32_t lt = tile >= L1 + 4 ? tile - L1 : NTT_LOG_TILE - L1;
+    uint32_t lt = tile >= L1 + 3 ? tile - L1 : NTT_LOG_TILE - L1;
     return lt > lines_log ? lines_log : lt;
 }
 
