@@ -215,18 +215,25 @@ def run_batched(args, cfg):
 
     # (b) batched: the first call autotunes the launch plan (graph branches, NTT tile
     # and radix, Merkle level schedule) and instantiates the graph; later calls replay it
+    # (the same request replays the graph the first call built: an all-0 plan is tuned per shape)
     t_tune = time.perf_counter()
-    streams = batch.launch(plan=dict(parse_plan(args.plan), n_streams=args.batch_streams) if args.plan
-                           else dict(n_streams=args.batch_streams))
+    request = (dict(parse_plan(args.plan), n_streams=args.batch_streams) if args.plan
+               else dict(n_streams=args.batch_streams))
+    streams = batch.launch(plan=request)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t_tune
     plan = batch.plan
+    shape_plans = {}
+    for j in shapes:
+        p = pc.batch_shape_plan(j["n_bits"], j["n_bits_ext"], j["n_cols"], arity)[1]
+        k = "tile 2^%d radix %d coop %d" % (p["ntt_tile_log"], 1 << p["ntt_radix_log"], p["coop_max_parents"])
+        shape_plans[k] = shape_plans.get(k, 0) + 1
     for _ in range(args.warmup):
-        batch.launch(plan=plan)
+        batch.launch(plan=request)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        batch.launch(plan=plan)
+        batch.launch(plan=request)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = elems / (dt / args.steps) / 1e9
@@ -252,6 +259,7 @@ def run_batched(args, cfg):
                    "n_instances": len(shapes), "lde_elements": elems, "arity": arity, "seed": cfg["seed"],
                    "hbm_resident_GB": round(need / 1e9, 2), "path": "batched (pil2hip_commit_batch)"},
         "batched": {"graph_branches": streams, "plan": plan, "autotune_s": round(t_tune, 3),
+                    "instances_per_shape_plan": shape_plans if not args.plan else None,
                     "roots_match_plain": match},
         "plain": {"value": round(plain_value, 4), "unit": "GElem/s", "ms_per_pass": round(dt_plain * 1e3, 3),
                   "path": "per-instance pil2hip_commit + root to host, one instance at a time",

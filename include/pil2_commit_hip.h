@@ -163,9 +163,12 @@ typedef struct Pil2CommitJob {
  * first, each to the least-loaded branch); each job's root is copied to
  * d_roots[4*i .. 4*i+4) (d_roots may be NULL). The instantiated graph is cached
  * per job list (pointers and shapes) and replayed on later calls.
- * n_streams = 0 autotunes the branch count on the first call for a job list
- * (1, 2, 4, 8, 16, each timed over one replay; these runs recompute the same
- * outputs); n_streams > 0 uses that many. The rest of the plan is the default
+ * n_streams = 0 autotunes the whole launch plan on the first call for a job
+ * list: the NTT block tile and in-tile radix per LDE shape and the cooperative
+ * tree-level threshold per tree shape (each shape measured once per process, on
+ * the first batch that has it; see pil2hip_batch_shape_plan), then the branch
+ * count (1, 2, 4, 8, 16, each timed over one replay). These runs recompute the
+ * same outputs. n_streams > 0 uses that many branches and the default plan
  * (Pil2CommitPlan below; pil2hip_commit_batch_plan takes an explicit one).
  * Returns the branch count used (> 0), 0 for an empty batch, or a negative error
  * code. Asynchronous like pil2hip_commit. */
@@ -174,7 +177,8 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
 
 /* Launch plan of a commit. Every plan computes the same values; they differ in
  * speed only. n_streams = 0 asks the batch autotuner for the branch count; the
- * other fields left 0 take their defaults.
+ * other fields left 0 take their defaults, except that a plan of all 0 asks the
+ * autotuner for everything.
  *   n_streams         graph branches, 1..16
  *   ntt_tile_log      largest NTT block tile, log2 values, 12..14 (a shape that
  *                     needs a larger tile gets it); smaller tiles fit more blocks per CU
@@ -186,13 +190,24 @@ typedef struct Pil2CommitPlan {
     uint32_t n_streams, ntt_tile_log, ntt_radix_log, coop_max_parents;
 } Pil2CommitPlan;
 
-/* pil2hip_commit_batch with an explicit plan: n_streams = 0 is tuned on the first
- * call for this job list (each branch count timed over one replay), other fields
- * left 0 take their defaults; plan == NULL = all 0. The plan used is written to
- * *chosen (may be NULL). Returns the branch count used, 0 for an empty batch, or
- * a negative code. */
+/* pil2hip_commit_batch with an explicit plan. A plan with every field 0 (or
+ * plan == NULL) is tuned completely, as pil2hip_commit_batch with n_streams = 0
+ * does; *chosen then holds the branch count and 0 in the other fields (they are
+ * per shape). Otherwise n_streams = 0 is tuned on the first call for this job
+ * list (each branch count timed over one replay) and the other fields left 0
+ * take their defaults. The plan used is written to *chosen (may be NULL).
+ * Returns the branch count used, 0 for an empty batch, or a negative code. */
 int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, const Pil2CommitPlan *plan,
                                   Pil2CommitPlan *chosen, uint64_t *d_roots, void *stream);
+
+/* The plan a fully tuned batch uses for a job of this shape: the tuner's NTT tile
+ * and radix for (nBits, nBitsExt, ceil(log2 nCols)) and its level threshold for
+ * (nBitsExt, arity), with defaults where a shape has not been measured yet
+ * (out->n_streams = 0). Returns 1 when both were measured, 0 otherwise, or a
+ * negative code. No counterpart in the reference: it reports the autotuner's
+ * per-shape choices. */
+int64_t pil2hip_batch_shape_plan(uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t arity,
+                                 Pil2CommitPlan *out);
 
 /* Destroys every cached batch graph. */
 void pil2hip_batch_release(void);

@@ -21,6 +21,10 @@ struct NttPlan {
 };
 
 extern "C" int ntt_init_device(void);
+// Nonzero signature of the launches ntt_launch_lde makes for this shape and plan
+// (plans with equal signatures launch the same kernels); 0 for a shape the plan
+// machinery does not cover (log_n < 4 uses a direct kernel).
+extern "C" uint64_t ntt_lde_plan_signature(uint32_t log_n, uint32_t log_ext, const NttPlan *plan);
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
                               uint64_t ncols, const NttTablesHost *th, const NttPlan *plan, hipStream_t st,
                               hipEvent_t *ev);

@@ -769,6 +769,37 @@ static uint32_t strided_lt(uint32_t L1, uint32_t lines_log, uint32_t tile)
     return lt > lines_log ? lines_log : lt;
 }
 
+// Launch geometry of an LDE of 2^log_n -> 2^log_ext rows under a plan: n2 * B <=
+// 2^tile values per block (tile <= 14), n1 = N / n2 <= 2^10; a shape that needs
+// n1 > 2^10 at the requested tile gets the smallest tile that fits. Returns
+// nonzero for a shape no plan covers (log_n >= 4 assumed).
+struct LdeGeometry {
+    uint32_t tile, rmax, L1, L2, lt_inv, lt_fwd;
+};
+static int lde_geometry(uint32_t log_n, uint32_t log_ext, const NttPlan *plan, LdeGeometry *g)
+{
+    uint32_t log_b = log_ext - log_n;
+    uint32_t tile = plan ? plan->tile_log : NTT_LOG_TILE, rmax = plan ? plan->radix_log : 4;
+    if (tile > NTT_LOG_TILE) tile = NTT_LOG_TILE;
+    if (tile < log_b + 4) tile = log_b + 4;
+    if (rmax < 1 || rmax > 4) rmax = 4;
+    uint32_t L2 = log_n;
+    if (L2 + log_b > tile) L2 = tile - log_b;
+    if (log_n - L2 > 10) L2 = log_n - 10;
+    uint32_t L1 = log_n - L2;
+    if (L2 + log_b > NTT_LOG_TILE || L2 < 4) return -2;  // n2*B <= 2^14 and >= 16 values per block line
+    *g = {tile, rmax, L1, L2, L1 ? strided_lt(L1, L2, tile) : 0, L1 ? strided_lt(L1, L2 + log_b, tile) : 0};
+    return 0;
+}
+
+extern "C" uint64_t ntt_lde_plan_signature(uint32_t log_n, uint32_t log_ext, const NttPlan *plan)
+{
+    LdeGeometry g;
+    if (log_n < 4 || log_ext < log_n || log_ext > NTT_MAX_LOG || lde_geometry(log_n, log_ext, plan, &g)) return 0;
+    // every launch parameter the plan influences (the tile only through L1/L2/lt)
+    return 1 + (g.rmax | (uint64_t)g.L2 << 8 | (uint64_t)g.lt_inv << 16 | (uint64_t)g.lt_fwd << 24);
+}
+
 extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *scratch, uint32_t log_n, uint32_t log_ext,
                               uint64_t ncols, const NttTablesHost *th, const NttPlan *plan, hipStream_t st,
                               hipEvent_t *ev)
@@ -794,21 +825,13 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
             if (ev) (void)hipEventRecord(ev[i], st);
         return (int)hipGetLastError();
     }
-    // plan: n2 * B <= 2^tile values per block (tile <= 14), n1 = N / n2 <= 2^10; a
-    // shape that needs n1 > 2^10 at the requested tile gets the smallest tile that fits
-    uint32_t tile = plan ? plan->tile_log : NTT_LOG_TILE, rmax = plan ? plan->radix_log : 4;
-    if (tile > NTT_LOG_TILE) tile = NTT_LOG_TILE;
-    if (tile < log_b + 4) tile = log_b + 4;
-    if (rmax < 1 || rmax > 4) rmax = 4;
-    uint32_t L2 = log_n;
-    if (L2 + log_b > tile) L2 = tile - log_b;
-    if (log_n - L2 > 10) L2 = log_n - 10;
-    uint32_t L1 = log_n - L2;
-    if (L2 + log_b > NTT_LOG_TILE || L2 < 4) return -2;  // n2*B <= 2^14 and >= 16 values per block line
+    LdeGeometry geo;
+    if (lde_geometry(log_n, log_ext, plan, &geo)) return -2;
+    uint32_t rmax = geo.rmax, L1 = geo.L1, L2 = geo.L2;
 
     // 1. strided inverse DIF over n1
     if (L1 > 0) {
-        uint32_t lt = strided_lt(L1, L2, tile);
+        uint32_t lt = geo.lt_inv;
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
@@ -826,7 +849,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
     // 3. strided forward DIT over n1
     if (L1 > 0) {
         uint32_t l2b = L2 + log_b;
-        uint32_t lt = strided_lt(L1, l2b, tile);
+        uint32_t lt = geo.lt_fwd;
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,

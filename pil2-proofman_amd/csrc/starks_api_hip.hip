@@ -11,7 +11,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -371,8 +374,10 @@ int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_
 // into d_roots, and the whole fork/join capture is instantiated once per job
 // list and plan and replayed. A branch count of 0 is tuned on the first call
 // (S in {1, 2, 4, 8, 16}, each candidate timed over one replay of the whole
-// batch after one warm replay); the other plan fields left 0 take their
-// defaults (tile 2^13, radix 8, cooperative levels up to 16,384 parents).
+// batch after one warm replay). A plan with every field 0 also takes each job's
+// NTT tile, radix and level threshold from the shape wisdom below (measured on
+// first sight of a shape); otherwise plan fields left 0 take their defaults
+// (tile 2^13, radix 8, cooperative levels up to 16,384 parents).
 struct BatchGraph {
     std::vector<uint64_t> key;
     Pil2CommitPlan plan;
@@ -404,6 +409,8 @@ static int64_t batch_resources()
     return 0;
 }
 
+static Pil2CommitPlan job_plan(const Pil2CommitJob &j, const Pil2CommitPlan &plan);
+
 static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, const Pil2CommitPlan &plan, uint64_t *d_roots,
                              hipGraphExec_t *out)
 {
@@ -433,7 +440,7 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, const Pil2Co
         const Pil2CommitJob &j = jobs[order[k]];
         hipStream_t st = g_batch_streams[assign[order[k]]];
         rc = commit_impl(j.d_lde, j.d_nodes, j.d_trace, j.d_scratch, j.arity, j.n_bits, j.n_bits_ext, j.n_cols, st,
-                         nullptr, plan);
+                         nullptr, job_plan(j, plan));
         if (!rc && d_roots) {
             uint64_t nt = pil2hip_tree_num_elements(1ull << j.n_bits_ext, j.arity);
             int e = p2_launch_copy_root(j.d_nodes + nt - 4, d_roots + 4 * order[k], st);
@@ -457,6 +464,132 @@ static int64_t batch_capture(const Pil2CommitJob *jobs, uint64_t n, const Pil2Co
     (void)hipGraphDestroy(graph);
     if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("graph instantiate: ") + hipGetErrorString(ee));
     return 0;
+}
+
+// ---- shape wisdom -------------------------------------------------------
+// The batch autotuner's per-shape choices: the NTT block tile and in-tile radix
+// per LDE shape (nBits, nBitsExt, ceil(log2 nCols)), and the cooperative-level
+// threshold per tree shape (nBitsExt, arity). Each shape is measured once per
+// process, eagerly (outside any graph) on the buffers of the first batch job that
+// has it, and every later batch reuses the entry. Plans whose launches coincide
+// for a shape (ntt_lde_plan_signature, level_schedule) are timed once.
+static std::map<uint64_t, NttPlan> g_lde_wisdom;
+static std::map<uint64_t, uint32_t> g_level_wisdom;
+static const uint32_t kLevelThresholds[] = {4096, 16384, 65536};
+
+static uint64_t lde_key(const Pil2CommitJob &j)
+{
+    uint64_t lc = 0;
+    while ((1ull << lc) < j.n_cols) lc++;
+    return j.n_bits | j.n_bits_ext << 8 | lc << 16;
+}
+static uint64_t level_key(const Pil2CommitJob &j) { return j.n_bits_ext | j.arity << 8; }
+
+// Number of tree levels a threshold sends to the cooperative kernel (the level
+// schedule p2_launch_tree_levels follows).
+static uint32_t level_schedule(uint64_t nleaves, uint64_t arity, uint64_t thr)
+{
+    uint32_t coop = 0;
+    for (uint64_t pending = nleaves; pending > 1; pending = (pending + arity - 1) / arity)
+        coop += (pending + arity - 1) / arity <= thr;
+    return coop;
+}
+
+// The plan job j runs under: `plan` itself, or with NTT fields 0 (a tuned batch)
+// the shape's wisdom (the defaults for a shape without an entry).
+static Pil2CommitPlan job_plan(const Pil2CommitJob &j, const Pil2CommitPlan &plan)
+{
+    if (plan.ntt_tile_log) return plan;
+    Pil2CommitPlan p = {plan.n_streams, kDefaultPlan.ntt_tile_log, kDefaultPlan.ntt_radix_log,
+                        kDefaultPlan.coop_max_parents};
+    auto a = g_lde_wisdom.find(lde_key(j));
+    if (a != g_lde_wisdom.end()) {
+        p.ntt_tile_log = a->second.tile_log;
+        p.ntt_radix_log = a->second.radix_log;
+    }
+    auto b = g_level_wisdom.find(level_key(j));
+    if (b != g_level_wisdom.end()) p.coop_max_parents = b->second;
+    return p;
+}
+
+// Best of two timed runs of f on st (after one untimed run).
+static int64_t time_best(const std::function<int64_t()> &f, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                         float *best)
+{
+    int64_t rc = f();
+    *best = 1e30f;
+    for (int k = 0; k < 2 && !rc; k++) {
+        HIP_TRY(hipEventRecord(e0, st), "tune event");
+        if ((rc = f())) break;
+        HIP_TRY(hipEventRecord(e1, st), "tune event");
+        HIP_TRY(hipEventSynchronize(e1), "tune sync");
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "tune time");
+        *best = std::min(*best, ms);
+    }
+    return rc;
+}
+
+// Measures every shape of the batch that has no wisdom yet. The runs write the
+// job's own LDE and tree (with the same values the commit then recomputes).
+static int64_t tune_shapes(const Pil2CommitJob *jobs, uint64_t n, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        const Pil2CommitJob &j = jobs[i];
+        uint64_t lk = lde_key(j);
+        if (!g_lde_wisdom.count(lk)) {
+            NttPlan best = {kDefaultPlan.ntt_tile_log, kDefaultPlan.ntt_radix_log};
+            float best_ms = 1e30f;
+            std::set<uint64_t> seen;
+            for (uint32_t tile = 12; tile <= NTT_LOG_TILE; tile++)
+                for (uint32_t radix = 2; radix <= 4; radix++) {
+                    NttPlan np = {tile, radix};
+                    uint64_t sig = ntt_lde_plan_signature((uint32_t)j.n_bits, (uint32_t)j.n_bits_ext, &np);
+                    if (!sig || !seen.insert(sig).second) continue;
+                    Pil2CommitPlan cp = {1, tile, radix, kDefaultPlan.coop_max_parents};
+                    float ms = 0;
+                    int64_t rc = time_best(
+                        [&] { return lde_impl(j.d_lde, j.d_trace, j.d_scratch, j.n_bits, j.n_bits_ext, j.n_cols, st,
+                                              nullptr, cp); },
+                        st, e0, e1, &ms);
+                    if (rc) return rc;
+                    if (ms < best_ms) best_ms = ms, best = np;
+                }
+            g_lde_wisdom[lk] = best;
+        }
+        uint64_t vk = level_key(j);
+        if (!g_level_wisdom.count(vk)) {
+            uint64_t NExt = 1ull << j.n_bits_ext;
+            uint32_t best = kDefaultPlan.coop_max_parents;
+            float best_ms = 1e30f;
+            std::set<uint32_t> seen;
+            for (uint32_t thr : kLevelThresholds) {
+                if (!seen.insert(level_schedule(NExt, j.arity, thr)).second) continue;
+                float ms = 0;
+                int64_t rc = time_best(
+                    [&]() -> int64_t {
+                        int e = p2_launch_tree_levels((uint32_t)(4 * j.arity), j.d_nodes, NExt, thr, st);
+                        return e ? fail(-100 - e, "tree level launch failed") : 0;
+                    },
+                    st, e0, e1, &ms);
+                if (rc) return rc;
+                if (ms < best_ms) best_ms = ms, best = thr;
+            }
+            g_level_wisdom[vk] = best;
+        }
+    }
+    return 0;
+}
+
+int64_t pil2hip_batch_shape_plan(uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t arity,
+                                 Pil2CommitPlan *out)
+{
+    if (!out) return fail(-13, "null plan");
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    Pil2CommitJob j = {nullptr, nullptr, nullptr, nullptr, nBits, nBitsExt, nCols, arity};
+    bool known = g_lde_wisdom.count(lde_key(j)) && g_level_wisdom.count(level_key(j));
+    *out = job_plan(j, Pil2CommitPlan{0, 0, 0, 0});
+    return known ? 1 : 0;
 }
 
 static bool plan_eq(const Pil2CommitPlan &a, const Pil2CommitPlan &b)
@@ -504,6 +637,14 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
         if (j.n_bits_ext < j.n_bits || j.n_bits_ext > NTT_MAX_LOG) return fail(-10, "bad nBits/nBitsExt in a job");
         if (!j.d_trace || !j.d_lde || !j.d_nodes || !j.d_scratch) return fail(-13, "null buffer in a job");
     }
+    // all fields 0: tune everything (branch count per job list, NTT tile and radix
+    // and level threshold per shape); otherwise plan fields left 0 take the defaults
+    bool tune_all = !want.n_streams && !want.ntt_tile_log && !want.ntt_radix_log && !want.coop_max_parents;
+    if (!tune_all) {
+        if (!want.ntt_tile_log) want.ntt_tile_log = kDefaultPlan.ntt_tile_log;
+        if (!want.ntt_radix_log) want.ntt_radix_log = kDefaultPlan.ntt_radix_log;
+        if (!want.coop_max_parents) want.coop_max_parents = kDefaultPlan.coop_max_parents;
+    }
     NttTablesHost *tb;
     if ((rc = ensure_tables(&tb))) return rc;  // allocations happen before any capture
     std::lock_guard<std::mutex> lk(g_batch_mu);
@@ -518,10 +659,9 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
     }
     hipStream_t st = (hipStream_t)stream;
     auto matches = [&](const Pil2CommitPlan &p) {
-        return (!want.n_streams || p.n_streams == want.n_streams) &&
-               (!want.ntt_tile_log || p.ntt_tile_log == want.ntt_tile_log) &&
-               (!want.ntt_radix_log || p.ntt_radix_log == want.ntt_radix_log) &&
-               (!want.coop_max_parents || p.coop_max_parents == want.coop_max_parents);
+        if (tune_all) return p.ntt_tile_log == 0;  // a graph built from shape wisdom
+        return (!want.n_streams || p.n_streams == want.n_streams) && p.ntt_tile_log == want.ntt_tile_log &&
+               p.ntt_radix_log == want.ntt_radix_log && p.coop_max_parents == want.coop_max_parents;
     };
     for (auto &g : g_batch_graphs)
         if (g.key == key && matches(g.plan)) {
@@ -529,11 +669,9 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
             if (chosen) *chosen = g.plan;
             return g.plan.n_streams;
         }
-    // defaults for the fields left 0; the branch count is then tuned when it was 0
+    // the branch count is tuned when it was 0; NTT fields 0 (tune_all) = per-shape wisdom
     Pil2CommitPlan cur = {want.n_streams ? want.n_streams : (uint32_t)std::min<uint64_t>(4, n_jobs),
-                          want.ntt_tile_log ? want.ntt_tile_log : kDefaultPlan.ntt_tile_log,
-                          want.ntt_radix_log ? want.ntt_radix_log : kDefaultPlan.ntt_radix_log,
-                          want.coop_max_parents ? want.coop_max_parents : (uint32_t)P2_COOP_MAX_PARENTS};
+                          want.ntt_tile_log, want.ntt_radix_log, want.coop_max_parents};
     std::vector<std::vector<Pil2CommitPlan>> stages;
     if (!want.n_streams) {
         std::vector<Pil2CommitPlan> c;
@@ -547,8 +685,9 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
     hipGraphExec_t best = nullptr;
     float best_ms = 1e30f;
     Pil2CommitPlan best_plan = cur;
+    if (tune_all) rc = tune_shapes(jobs, n_jobs, st, e0, e1);  // eager runs, before any capture
     bool tuning = !stages.empty();
-    if (!tuning) {
+    if (!tuning && !rc) {
         float ms = 0;
         rc = batch_try(jobs, n_jobs, cur, d_roots, st, false, e0, e1, &best, &ms);
     }
@@ -950,7 +1089,7 @@ static int64_t write_custom_commit_impl(void *root, uint64_t arity, uint64_t nBi
         if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(-25, "stream sync failed");
         if (!rc) rc = write_file(bufferFile, root, (const uint64_t *)buffer, tsz, lde.data(), lsz, nodes.data(), nTree);
     }
-    hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st);  // also after an error: the buffers are freed next
     cleanup();
     return rc;
 }

@@ -57,6 +57,7 @@ _SIGS = {
     "pil2hip_compute_q": (_i64, [_vp, _vp, _vp, _u64, _u64, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_commit_batch_plan": (_i64, [_vp, _u64, _vp, _vp, _vp, _vp]),
+    "pil2hip_batch_shape_plan": (_i64, [_u64, _u64, _u64, _u64, _vp]),
     "pil2hip_batch_release": (None, []),
     "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
     "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
@@ -481,8 +482,10 @@ class CommitBatch:
 
     def launch(self, n_streams=0, stream=None, plan=None):
         """Enqueue the batch; returns the number of graph branches used. `plan`
-        (a dict of Pil2CommitPlan fields, missing or 0 = tune) selects
-        pil2hip_commit_batch_plan; the plan used is kept in self.plan."""
+        (a dict of Pil2CommitPlan fields: all missing or 0 = tune everything per
+        shape; otherwise n_streams 0 = tune the branch count and other fields 0 =
+        defaults) selects pil2hip_commit_batch_plan; the plan used is kept in
+        self.plan."""
         if plan is None:
             rc = _lib.pil2hip_commit_batch(ctypes.cast(self._arr, _vp), len(self.jobs), n_streams,
                                            _ptr(self.roots), _stream(stream))
@@ -500,6 +503,15 @@ class CommitBatch:
 
     def roots_host(self):
         return to_host(self.roots)[:4 * len(self.jobs)].reshape(-1, 4)
+
+
+def batch_shape_plan(n_bits, n_bits_ext, n_cols, arity):
+    """(measured, plan dict) the tuned batch path uses for a job of this shape."""
+    out = Pil2CommitPlan()
+    rc = _lib.pil2hip_batch_shape_plan(n_bits, n_bits_ext, n_cols, arity, ctypes.byref(out))
+    if rc < 0:
+        _check(rc)
+    return bool(rc), out.as_dict()
 
 
 def batch_release():

@@ -67,3 +67,18 @@ def test_set_gpu_mode_reports_gpu_backend(lib):
     f.argtypes = [ctypes.c_bool]
     assert f(True) is True
     assert f(False) is False
+
+
+def test_batch_shape_plan_defaults_before_tuning(lib):
+    """A shape the batch autotuner has not measured reports 0 and the default plan
+    (pure host logic: no device call)."""
+    class Plan(ctypes.Structure):
+        _fields_ = [("n_streams", ctypes.c_uint32), ("ntt_tile_log", ctypes.c_uint32),
+                    ("ntt_radix_log", ctypes.c_uint32), ("coop_max_parents", ctypes.c_uint32)]
+    f = lib.pil2hip_batch_shape_plan
+    f.restype = ctypes.c_int64
+    f.argtypes = [ctypes.c_uint64] * 4 + [ctypes.c_void_p]
+    p = Plan()
+    assert f(20, 22, 100, 3, ctypes.byref(p)) == 0
+    assert (p.n_streams, p.ntt_tile_log, p.ntt_radix_log, p.coop_max_parents) == (0, 13, 3, 16384)
+    assert f(20, 22, 100, 3, None) < 0

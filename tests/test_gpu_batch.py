@@ -165,27 +165,36 @@ def test_batch_plans_are_bit_identical(pc, oracle):
     assert np.array_equal(want[-3][1][-4:], oracle.merkletree(oracle.lde(trace, 1 << nbe), 3)[-4:])
 
 
-@pytest.mark.skip(reason="host segmentation fault in the first run of the full-plan tuner (tile, radix and "
-                         "level-schedule stages) on this 10-job list; cause not found, so those tuning stages "
-                         "are switched off and fields left 0 take their defaults")
 def test_batch_plan_autotune(pc):
-    """Fields left 0 are tuned; the chosen plan is valid and replays from the cache."""
+    """A plan of all 0 tunes the NTT tile and radix per LDE shape, the level
+    threshold per tree shape and the branch count per job list. The outputs equal
+    the per-instance path's, every shape gets a valid measured plan, and the tuned
+    graph replays from the cache (pil2hip_commit_batch with n_streams = 0 too)."""
+    print("autotune shapes", SHAPES, flush=True)
     jobs = _jobs(pc, SHAPES, 99)
     want = [_plain(pc, j, 3) for j in jobs]
     b = pc.CommitBatch(jobs, arity=3)
     s = b.launch(plan=dict(n_streams=0, ntt_tile_log=0, ntt_radix_log=0, coop_max_parents=0))
     _sync()
-    p = b.plan
-    assert s == p["n_streams"] >= 1 and 12 <= p["ntt_tile_log"] <= 14 and p["ntt_radix_log"] in (3, 4)
-    assert p["coop_max_parents"] in (4096, 16384, 65536)
+    assert s >= 1 and b.plan == dict(n_streams=s, ntt_tile_log=0, ntt_radix_log=0, coop_max_parents=0)
+    for k, j in enumerate(b.jobs):
+        assert np.array_equal(pc.to_host(j["lde"]), want[k][0]), ("lde", k)
+        assert np.array_equal(pc.to_host(j["nodes"]), want[k][1]), ("nodes", k)
+    for nb, nbe, nc in SHAPES:
+        known, p = pc.batch_shape_plan(nb, nbe, nc, 3)
+        assert known, (nb, nbe, nc)
+        assert 12 <= p["ntt_tile_log"] <= 14 and p["ntt_radix_log"] in (2, 3, 4), p
+        assert p["coop_max_parents"] in (4096, 16384, 65536), p
     for rep in range(2):
         for j in b.jobs:
             j["nodes"].zero_()
+            j["lde"].zero_()
         _sync()
-        assert b.launch(plan=p) == s
+        assert b.launch(n_streams=0) == s
         _sync()
         for k, j in enumerate(b.jobs):
-            assert np.array_equal(pc.to_host(j["nodes"]), want[k][1]), (rep, k)
+            assert np.array_equal(pc.to_host(j["lde"]), want[k][0]), (rep, "lde", k)
+            assert np.array_equal(pc.to_host(j["nodes"]), want[k][1]), (rep, "nodes", k)
     bad = pc.Pil2CommitPlan(0, 15, 0, 0)
     assert pc.lib.pil2hip_commit_batch_plan(pc.ctypes.cast(b._arr, pc._vp), len(b.jobs), pc.ctypes.byref(bad),
                                             None, pc._ptr(b.roots), pc._stream()) < 0
