@@ -340,6 +340,28 @@ int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t 
  * w = 2^tBits, h = degree / w, 3-word elements. */
 int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t degree, uint64_t tBits, void *stream);
 
+/* The FRI layer loop of gen_proof: for s = 0 .. nSteps-1, step s > 0 folds
+ * (FRI::fold, fri.hpp:32-99) d_pol (s = 1) or d_work (later steps, in place) into
+ * d_work with the challenge at d_challenges[3s .. 3s+3) (device memory, read when
+ * the fold runs), and step s < nSteps-1 commits its layer (FRI::merkelize,
+ * fri.hpp:102-109): getTransposed into d_sources[s] (2^steps[s] cubic values)
+ * and merkelize into d_nodes[s] (pil2hip_tree_num_elements(2^steps[s+1], arity)
+ * words), i.e. a tree of 2^steps[s+1] rows of 3*2^(steps[s]-steps[s+1]) words.
+ * steps[0] must equal nBitsExt and each step drops 1 .. 5 bits
+ * (stark_struct.rs:162-166 gives the schedule). graph = true captures the whole
+ * sequence once per argument set (all pointers, steps, arity) and replays the
+ * cached graph on later calls, so new challenges written into d_challenges are
+ * used without a new capture; graph = false enqueues the kernels directly (for
+ * a caller's own capture). d_work needs 3*2^steps[1] words. Asynchronous on
+ * `stream`. No counterpart in the reference API: gen_proof (starks_api.hpp:122)
+ * runs this loop internally. */
+int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_challenges, uint64_t nBitsExt,
+                           const uint64_t *steps, uint64_t nSteps, uint64_t arity, uint64_t *const *d_sources,
+                           uint64_t *const *d_nodes, bool graph, void *stream);
+
+/* Destroys every cached FRI layer graph. */
+void pil2hip_fri_release(void);
+
 /* Column-major -> row-major copy (e.g. to hand an LDE back in the CPU layout). */
 int64_t pil2hip_transpose_cm2rm(uint64_t *d_out, const uint64_t *d_in, uint64_t rows, uint64_t cols,
                                 uint64_t inColStride, uint64_t outRowStride, void *stream);

@@ -12,16 +12,25 @@
 #include "gl_field.h"
 #include "fri_kernels.h"
 
+// The challenge is (ch0, ch1, ch2), or the three words at chp when chp is not
+// null (device memory read at run time, so a captured graph folds with the
+// buffer's current contents).
 template <int LOGX>
 __global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, const uint64_t *__restrict__ pol,
                                                   uint32_t cur_bits, uint64_t shift_inv, uint64_t w_inv,
-                                                  uint64_t ch0, uint64_t ch1, uint64_t ch2, uint64_t wx_inv,
+                                                  uint64_t ch0, uint64_t ch1, uint64_t ch2,
+                                                  const uint64_t *__restrict__ chp, uint64_t wx_inv,
                                                   uint64_t nx_inv)
 {
     constexpr int NX = 1 << LOGX;
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t n2 = 1ull << cur_bits;
     if (g >= n2) return;
+    if (chp) {
+        ch0 = gl_canon(chp[0]);
+        ch1 = gl_canon(chp[1]);
+        ch2 = gl_canon(chp[2]);
+    }
     uint64_t a[3][NX];
 #pragma unroll
     for (int i = 0; i < NX; i++) {
@@ -88,6 +97,13 @@ __global__ __launch_bounds__(256) void k_fri_transpose(uint64_t *__restrict__ au
 extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challenge, uint64_t n_bits_ext,
                                uint64_t prev_bits, uint64_t cur_bits, hipStream_t st)
 {
+    return fri_launch_fold2(out, pol, challenge, nullptr, n_bits_ext, prev_bits, cur_bits, st);
+}
+
+extern "C" int fri_launch_fold2(uint64_t *out, const uint64_t *pol, const uint64_t *challenge,
+                                const uint64_t *d_challenge, uint64_t n_bits_ext, uint64_t prev_bits,
+                                uint64_t cur_bits, hipStream_t st)
+{
     if (cur_bits > prev_bits || prev_bits > n_bits_ext || prev_bits > 32) return -1;
     uint32_t logx = (uint32_t)(prev_bits - cur_bits);
     uint64_t shift = 7;
@@ -107,13 +123,14 @@ extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_
     uint64_t w_inv = gl_inv(W[prev_bits]);
     uint64_t wx_inv = gl_inv(W[logx]);
     uint64_t nx_inv = gl_inv((1ull << logx) % GL_P);
-    uint64_t c0 = gl_canon(challenge[0]), c1 = gl_canon(challenge[1]), c2 = gl_canon(challenge[2]);
+    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    if (!d_challenge) c0 = gl_canon(challenge[0]), c1 = gl_canon(challenge[1]), c2 = gl_canon(challenge[2]);
     uint64_t n2 = 1ull << cur_bits;
     dim3 g((unsigned)((n2 + 255) / 256));
 #define FOLD_CASE(L)                                                                                        \
     case L:                                                                                                 \
         hipLaunchKernelGGL(k_fri_fold<L>, g, dim3(256), 0, st, out, pol, (uint32_t)cur_bits, shift_inv,     \
-                           w_inv, c0, c1, c2, wx_inv, nx_inv);                                              \
+                           w_inv, c0, c1, c2, d_challenge, wx_inv, nx_inv);                                 \
         break;
     switch (logx) {
         FOLD_CASE(1)

@@ -991,6 +991,106 @@ int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t d
     return 0;
 }
 
+// ---------------------------------------------------------- FRI layer sequence --
+// The FRI loop of gen_proof: fold step s > 0 (fri.hpp:32-99) into d_work, then,
+// below the last step, getTransposed + merkelize of the layer (fri.hpp:102-109).
+// Every launch reads its inputs from device memory (the challenges included), so
+// the whole sequence is captured once per argument set and replayed: the layers
+// are a chain of short, latency-bound kernels, and one graph launch removes the
+// host launch gaps between them.
+struct FriGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+};
+static std::vector<FriGraph> g_fri_graphs;
+static std::mutex g_fri_mu;
+static hipStream_t g_fri_capture = nullptr;
+
+static int64_t fri_enqueue(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_ch, uint64_t nbe,
+                           const uint64_t *steps, uint64_t n, uint64_t arity, uint64_t *const *d_sources,
+                           uint64_t *const *d_nodes, hipStream_t st)
+{
+    uint32_t width = (uint32_t)(4 * arity);
+    for (uint64_t s = 0; s < n; s++) {
+        if (s > 0) {
+            int e = fri_launch_fold2(d_work, s == 1 ? d_pol : d_work, nullptr, d_ch + 3 * s, nbe, steps[s - 1],
+                                     steps[s], st);
+            if (e) return fail(-100 - e, "fold launch failed");
+        }
+        if (s + 1 < n) {
+            uint64_t rows = 1ull << steps[s + 1], cols = 3ull << (steps[s] - steps[s + 1]);
+            int e = fri_launch_transpose(d_sources[s], s == 0 ? d_pol : d_work, 1ull << steps[s],
+                                         (uint32_t)steps[s + 1], st);
+            if (!e) e = p2_launch_leaves(width, d_sources[s], cols, 1, cols, rows, d_nodes[s], st);
+            if (!e) e = p2_launch_tree_levels(width, d_nodes[s], rows, P2_COOP_MAX_PARENTS, st);
+            if (e) return fail(-100 - e, "layer commit launch failed");
+        }
+    }
+    return 0;
+}
+
+int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_challenges, uint64_t nBitsExt,
+                           const uint64_t *steps, uint64_t nSteps, uint64_t arity, uint64_t *const *d_sources,
+                           uint64_t *const *d_nodes, bool graph, void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (!steps || nSteps == 0) return fail(-16, "no FRI steps");
+    if (steps[0] != nBitsExt || nBitsExt > 32) return fail(-16, "steps[0] must equal nBitsExt (<= 32)");
+    for (uint64_t s = 1; s < nSteps; s++)
+        if (steps[s] >= steps[s - 1] || steps[s - 1] - steps[s] > 5)
+            return fail(-17, "FRI steps must decrease by 1 .. 5 bits");
+    if (!d_pol || (nSteps > 1 && (!d_work || !d_challenges))) return fail(-13, "null buffer");
+    if (nSteps > 1 && (!d_sources || !d_nodes)) return fail(-13, "null layer buffers");
+    for (uint64_t s = 0; s + 1 < nSteps; s++)
+        if (!d_sources[s] || !d_nodes[s]) return fail(-13, "null layer buffer");
+    hipStream_t st = (hipStream_t)stream;
+    if (!graph) return fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, st);
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)d_pol, (uint64_t)(uintptr_t)d_work,
+                                 (uint64_t)(uintptr_t)d_challenges, nBitsExt, arity};
+    for (uint64_t s = 0; s < nSteps; s++) key.push_back(steps[s]);
+    for (uint64_t s = 0; s + 1 < nSteps; s++) {
+        key.push_back((uint64_t)(uintptr_t)d_sources[s]);
+        key.push_back((uint64_t)(uintptr_t)d_nodes[s]);
+    }
+    std::lock_guard<std::mutex> lk(g_fri_mu);
+    for (auto &g : g_fri_graphs)
+        if (g.key == key) {
+            HIP_TRY(hipGraphLaunch(g.exec, st), "FRI graph launch");
+            return 0;
+        }
+    if (!g_fri_capture) HIP_TRY(hipStreamCreateWithFlags(&g_fri_capture, hipStreamNonBlocking), "FRI capture stream");
+    HIP_TRY(hipStreamBeginCapture(g_fri_capture, hipStreamCaptureModeThreadLocal), "FRI begin capture");
+    t_capturing = true;
+    rc = fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, g_fri_capture);
+    t_capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(g_fri_capture, &g);
+    if (rc || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(-100 - (int64_t)ee, std::string("FRI end capture: ") + hipGetErrorString(ee));
+    }
+    hipGraphExec_t ex = nullptr;
+    ee = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("FRI graph instantiate: ") + hipGetErrorString(ee));
+    if (g_fri_graphs.size() >= 8) {  // small cache: drop the oldest graph
+        (void)hipGraphExecDestroy(g_fri_graphs.front().exec);
+        g_fri_graphs.erase(g_fri_graphs.begin());
+    }
+    g_fri_graphs.push_back({key, ex});
+    HIP_TRY(hipGraphLaunch(ex, st), "FRI graph launch");
+    return 0;
+}
+
+void pil2hip_fri_release(void)
+{
+    std::lock_guard<std::mutex> lk(g_fri_mu);
+    for (auto &g : g_fri_graphs) (void)hipGraphExecDestroy(g.exec);
+    g_fri_graphs.clear();
+}
+
 int64_t pil2hip_transpose_cm2rm(uint64_t *d_out, const uint64_t *d_in, uint64_t rows, uint64_t cols,
                                 uint64_t inColStride, uint64_t outRowStride, void *stream)
 {

@@ -58,6 +58,8 @@ _SIGS = {
     "pil2hip_commit_batch": (_i64, [_vp, _u64, ctypes.c_uint32, _vp, _vp]),
     "pil2hip_commit_batch_plan": (_i64, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pil2hip_batch_shape_plan": (_i64, [_u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_fri_layers": (_i64, [_vp, _vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, ctypes.c_bool, _vp]),
+    "pil2hip_fri_release": (None, []),
     "pil2hip_batch_release": (None, []),
     "pil2hip_group_proofs": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
     "pil2hip_proof_levels": (_u64, [_u64, _u64, _u64]),
@@ -364,6 +366,26 @@ class FRI:
         _check(_lib.pil2hip_fri_fold(_ptr(dst), _ptr(pol), ch.ctypes.data_as(_vp), step, n_bits_ext,
                                      prev_bits, current_bits, _stream(stream)))
         return dst
+
+    @staticmethod
+    def layers(pol, work, challenges, n_bits_ext, steps, trees, arity=3, graph=True, stream=None):
+        """The FRI layer loop (pil2hip_fri_layers): fold step s > 0 with the device
+        challenges[3s:3s+3] into `work`, commit layer s < len(steps)-1 into
+        trees[s] (MerkleTreeGL of 2^steps[s+1] rows; its source and nodes tensors
+        are allocated here when absent). graph=True replays one cached hipGraph."""
+        n = len(steps)
+        for s_ in range(n - 1):
+            t = trees[s_]
+            if t.source is None:
+                t.source = empty(3 << steps[s_], device=pol.device)
+            if t.nodes is None:
+                t.nodes = empty(t.num_nodes, device=pol.device)
+            t.row_stride, t.col_stride = t.width, 1
+        st = (ctypes.c_uint64 * n)(*steps)
+        srcs = (_vp * max(n - 1, 1))(*[_ptr(t.source) for t in trees[:n - 1]])
+        nds = (_vp * max(n - 1, 1))(*[_ptr(t.nodes) for t in trees[:n - 1]])
+        _check(_lib.pil2hip_fri_layers(_ptr(pol), _ptr(work), _ptr(challenges), n_bits_ext, st, n, arity,
+                                       srcs, nds, graph, _stream(stream)))
 
     @staticmethod
     def prove_queries(tree, queries, current_bits, stream=None):

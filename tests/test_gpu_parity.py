@@ -305,6 +305,56 @@ def test_fri_fold_all_layers(pc, oracle, nbe):
                 assert (proofs[qi] == exp).all(), ("fri query", s, q)
 
 
+def _fri_per_layer(pc, pol_host, chal, nbe, steps):
+    """Layer roots and the final polynomial from the per-call entry points."""
+    d = pc.to_device(pol_host)
+    roots = []
+    for s in range(len(steps)):
+        if s > 0:
+            pc.FRI.fold(s, d, chal[s], nbe, steps[s - 1], steps[s])
+        if s < len(steps) - 1:
+            t = pc.MerkleTreeGL(3, 2, False, 1 << steps[s + 1], 3 << (steps[s] - steps[s + 1]))
+            roots.append(pc.FRI.merkelize(t, d, steps[s], steps[s + 1]))
+    _sync()
+    return roots, pc.to_host(d)[:3 << steps[-1]]
+
+
+@pytest.mark.parametrize("nbe", [12, 22])
+def test_fri_layers_graph(pc, oracle, nbe):
+    """pil2hip_fri_layers (the whole fold + layer-commit loop, graph-captured) equals
+    the per-call path: every layer root and the final polynomial, for the graph's
+    first run, for replays after new challenges are written into the device
+    buffer (eager work and a sync between replays), and for the eager form.
+    nbe = 12 is also checked against the oracle."""
+    steps = fri_steps(nbe)
+    pol = datagen.uniform(3 << nbe, 0x5EED0004 + nbe)
+    d_pol = pc.to_device(pol)
+    work = pc.empty(3 << steps[1])
+    trees = [pc.MerkleTreeGL(3, 2, False, 1 << steps[s + 1], 3 << (steps[s] - steps[s + 1]))
+             for s in range(len(steps) - 1)]
+    d_ch = pc.empty(3 * len(steps))
+    for rep in range(3):
+        chal = datagen.challenges(len(steps), 0x5EED0004 + 1 + 17 * rep)
+        print("fri layers graph", nbe, "replay", rep, flush=True)
+        d_ch.copy_(pc.to_device(np.asarray(chal, dtype=np.uint64).reshape(-1)))
+        for t in trees:
+            if t.nodes is not None:
+                t.nodes.zero_()
+        work.zero_()
+        _sync()
+        pc.FRI.layers(d_pol, work, d_ch, nbe, steps, trees, graph=(rep < 2))
+        _sync()
+        roots, final = _fri_per_layer(pc, pol, chal, nbe, steps)
+        for s, t in enumerate(trees):
+            assert (t.get_root() == roots[s]).all(), (rep, "root", s)
+        assert (pc.to_host(work)[:3 << steps[-1]] == final).all(), (rep, "final layer")
+        if nbe == 12 and rep == 0:
+            host = pol.copy()
+            for s in range(1, len(steps)):
+                host = oracle.fri_fold(host, chal[s], s, nbe, steps[s - 1], steps[s])
+            assert (final == host).all()
+
+
 # ------------------------------------------------------------- openings --
 @pytest.mark.parametrize("arity,llv", [(3, 2), (4, 2), (2, 0)])
 def test_group_proofs_match_oracle(pc, oracle, arity, llv):
