@@ -459,38 +459,47 @@ def run_serving(args, cfg):
             got += 1
     cal_s = time.perf_counter() - t0
     capacity = sum(elems[:n_cal]) / cal_s  # elements per second
-    lam = cfg["load"] * capacity / (sum(elems) / n_jobs)  # jobs per second
-    rng = np.random.default_rng(cfg["seed"])
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
+    mean_elems = sum(elems) / n_jobs
 
-    lat, qdelay, done, bad = [], [], 0, 0
-    t_first = t_last = None
-    start = time.perf_counter()
-    base_us = eng.now_us()
-    k = 0
-    last_progress = start
-    while done < n_jobs:
-        now = time.perf_counter() - start
-        while k < n_jobs and arrivals[k] <= now:
-            submit(k)
-            k += 1
-        wait_ms = 0 if k >= n_jobs else max(0, int((arrivals[k] - (time.perf_counter() - start)) * 1e3))
-        for rec in eng.poll(max_n=256, timeout_ms=min(wait_ms, 50) if k < n_jobs else 100):
-            if rec["status"] != 0:
-                bad += 1
-            lat.append((rec["t_done_us"] - rec["t_submit_us"]) / 1e3)
-            qdelay.append((rec["t_admit_us"] - rec["t_submit_us"]) / 1e3)
-            t_first = rec["t_submit_us"] if t_first is None else min(t_first, rec["t_submit_us"])
-            t_last = rec["t_done_us"] if t_last is None else max(t_last, rec["t_done_us"])
-            eng.forget(rec["job_id"])
-            done += 1
-        if time.perf_counter() - last_progress > 20:
-            last_progress = time.perf_counter()
-            print("serving: %d/%d submitted, %d done" % (k, n_jobs, done), file=sys.stderr, flush=True)
-    span_s = (t_last - min(t_first, base_us)) / 1e6
+    def open_loop(jobs, load, seed):
+        """Poisson arrivals of `jobs` (indices) at `load` x capacity; returns latency
+        and queue-delay arrays (ms), the span (s) and the failed-job count."""
+        lam_ = load * capacity / mean_elems  # jobs per second
+        arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam_, len(jobs)))
+        lat_, qd_, done_, bad_, k_ = [], [], 0, 0, 0
+        t_first_ = t_last_ = None
+        start = time.perf_counter()
+        base_us = eng.now_us()
+        last_progress = start
+        while done_ < len(jobs):
+            now = time.perf_counter() - start
+            while k_ < len(jobs) and arr[k_] <= now:
+                submit(jobs[k_])
+                k_ += 1
+            wait_ms = 0 if k_ >= len(jobs) else max(0, int((arr[k_] - (time.perf_counter() - start)) * 1e3))
+            for rec in eng.poll(max_n=256, timeout_ms=min(wait_ms, 50) if k_ < len(jobs) else 100):
+                if rec["status"] != 0:
+                    bad_ += 1
+                lat_.append((rec["t_done_us"] - rec["t_submit_us"]) / 1e3)
+                qd_.append((rec["t_admit_us"] - rec["t_submit_us"]) / 1e3)
+                t_first_ = rec["t_submit_us"] if t_first_ is None else min(t_first_, rec["t_submit_us"])
+                t_last_ = rec["t_done_us"] if t_last_ is None else max(t_last_, rec["t_done_us"])
+                eng.forget(rec["job_id"])
+                done_ += 1
+            if time.perf_counter() - last_progress > 20:
+                last_progress = time.perf_counter()
+                print("serving: %d/%d submitted, %d done" % (k_, len(jobs), done_), file=sys.stderr, flush=True)
+        return np.array(lat_), np.array(qd_), (t_last_ - min(t_first_, base_us)) / 1e6, bad_, lam_
+
+    # the measured stream: every job at the configured offered load
+    lat, qdelay, span_s, bad, lam = open_loop(list(range(n_jobs)), cfg["load"], cfg["seed"])
+    # the same server below capacity: the first jobs of the stream again at 0.7x, where
+    # latency is the service time plus little queueing
+    n_low = min(400, n_jobs)
+    lat_lo, qd_lo, span_lo, bad_lo, lam_lo = open_loop(list(range(n_low)), 0.7, cfg["seed"] + 1)
+    bad += bad_lo
     st = eng.stats()
     eng.close()
-    lat = np.array(lat)
     value = sum(elems) / span_s / 1e9
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -507,6 +516,12 @@ def run_serving(args, cfg):
                        "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
         "queue_delay_ms": {"p50": round(float(np.percentile(qdelay, 50)), 3),
                            "p99": round(float(np.percentile(qdelay, 99)), 3)},
+        "below_capacity": {"offered_load": 0.7, "jobs": n_low, "arrival_rate_jobs_s": round(lam_lo, 3),
+                           "latency_ms": {"p50": round(float(np.percentile(lat_lo, 50)), 3),
+                                          "p99": round(float(np.percentile(lat_lo, 99)), 3)},
+                           "queue_delay_ms": {"p50": round(float(np.percentile(qd_lo, 50)), 3),
+                                              "p99": round(float(np.percentile(qd_lo, 99)), 3)},
+                           "GElem_s": round(sum(elems[:n_low]) / span_lo / 1e9, 4)},
         "capacity_GElem_s": round(capacity / 1e9, 4), "arrival_rate_jobs_s": round(lam, 3),
         "lde_elements": int(sum(elems)), "failed_jobs": bad,
         "server": {k2: st[k2] for k2 in ("admitted", "completed", "failed", "peak_bytes", "peak_running")},
