@@ -9,7 +9,7 @@
 #define NTT_MAX_LOG 24                     // largest transform handled (2^24 rows)
 
 struct NttTablesHost {
-    uint64_t *t14f, *t14i;                 // 2^14 entries each: w_{2^14}^(+-y)
+    uint64_t *tkf, *tki;                   // 2^15 entries each: [2^k + y] = w_{2^k}^(+-y), k <= 14
     uint64_t *twl_f, *twh_f, *twl_i, *twh_i;  // 4096 entries each
     uint64_t *shl, *shh;                   // 4096 entries each
 };

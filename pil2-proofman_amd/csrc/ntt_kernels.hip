@@ -31,11 +31,13 @@
 
 // --------------------------------------------------------------- tables ----
 // All roots derive from the reference's chain W[k] (goldilocks_base_field.cpp:11-44).
-//   t14f[y] = w_{2^14}^y, t14i[y] = w_{2^14}^-y        (y < 2^14)    inside tiles
+//   tkf[2^k + y] = w_{2^k}^y, tki[2^k + y] = w_{2^k}^-y  (k <= 14, y < 2^k)  inside tiles:
+//        one compact table per round size, so a round's twiddles of 2^k <= 2^12
+//        entries fit the 32 KiB L1 instead of being spread over a 2^14-entry table
 //   twl/twh: w_{2^24}^x = lo[x & 4095] * hi[x >> 12]   (x < 2^24)    between passes
 //   shl/shh: 7^e = lo[e & 4095] * hi[e >> 12]                        coset shift
 struct NttTables {
-    const uint64_t *t14f, *t14i;
+    const uint64_t *tkf, *tki;
     const uint64_t *twl_f, *twh_f, *twl_i, *twh_i;
     const uint64_t *shl, *shh;
 };
@@ -252,7 +254,7 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = tb.t14i[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = tb.tki[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
             twiddle_rest<R>(v, w);
         }
 #pragma unroll
@@ -278,7 +280,7 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = (INV ? tb.t14i : tb.t14f)[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = (INV ? tb.tki : tb.tkf)[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
             twiddle_rest<R>(v, w);
         }
         dft_dit<R, INV>(v);
@@ -402,7 +404,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = tb.t14i[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = tb.tki[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
             twiddle_rest<R>(v, w);
         }
         if (last) {  // coefficient k of the line times 7^e / N, e = bitrev(m*n2 + k)
@@ -476,7 +478,7 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
         if (p.o_lo) {
             uint64_t w[R];
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = (INV ? tb.t14i : tb.t14f)[(p.o_lo * bitrev32(j, LR)) << (14 - q - LR)];
+            for (int j = 1; j < R; j++) w[j] = (INV ? tb.tki : tb.tkf)[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
             twiddle_rest<R>(v[gg], w);
         }
         dft_dit<R, INV>(v[gg]);
@@ -805,7 +807,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
                               hipEvent_t *ev)
 {
     // ev (optional): events recorded after passes 0..3 (ev[0..3]) for phase timing
-    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
     uint64_t N = 1ull << log_n;
     if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
     uint32_t log_b = log_ext - log_n;
@@ -863,7 +865,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
                                     uint32_t log_n, uint32_t log_ext, uint32_t q_deg, uint32_t q_dim,
                                     const NttTablesHost *th, hipStream_t st)
 {
-    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
     if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
     uint32_t log_b = log_ext - log_n;
     if (q_deg == 0 || q_dim == 0 || q_deg > (1u << log_b)) return -3;
@@ -937,7 +939,7 @@ extern "C" int ntt_launch_ntt_tiny(uint64_t *dst, const uint64_t *src, uint64_t 
 extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, uint64_t ncols, int inverse,
                               const NttTablesHost *th, hipStream_t st)
 {
-    NttTables tb = {th->t14f, th->t14i, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
     if (lm > NTT_MAX_LOG) return -3;
     uint64_t M = 1ull << lm;
     uint64_t scale = inverse ? gl_inv(M % GL_P) : 1;

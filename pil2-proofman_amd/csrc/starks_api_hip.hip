@@ -66,18 +66,19 @@ static int64_t ensure_tables(NttTablesHost **out)
     if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_tables_ready[dev]) {
-        const size_t nr = (size_t)1 << NTT_LOG_TILE, nt = 4096;
+        const size_t nr = (size_t)2 << NTT_LOG_TILE, nt = 4096;
         std::vector<uint64_t> h(2 * nr + 6 * nt);
         uint64_t *rf = h.data(), *ri = rf + nr, *t = ri + nr;
-        {  // rf[y] = w_{2^14}^y, ri[y] = w_{2^14}^-y
-            uint64_t w = kRoots[NTT_LOG_TILE], wi = gl_inv(w), a = 1, b = 1;
-            for (size_t y = 0; y < nr; y++) {
-                rf[y] = a;
-                ri[y] = b;
+        for (uint32_t k = 0; k <= NTT_LOG_TILE; k++) {  // rf[2^k + y] = w_{2^k}^y, ri[2^k + y] = w_{2^k}^-y
+            uint64_t w = kRoots[k], wi = gl_inv(w), a = 1, b = 1;
+            for (size_t y = 0; y < ((size_t)1 << k); y++) {
+                rf[((size_t)1 << k) + y] = a;
+                ri[((size_t)1 << k) + y] = b;
                 a = gl_mul(a, w);
                 b = gl_mul(b, wi);
             }
         }
+        rf[0] = ri[0] = 1;  // unused slot
         int ie = ntt_init_device();
         if (ie) return fail(-100 - ie, "NTT device init failed");
         uint64_t w24 = kRoots[24], w24i = gl_inv(w24);
@@ -95,8 +96,8 @@ static int64_t ensure_tables(NttTablesHost **out)
         HIP_TRY(hipMalloc(&dptr, h.size() * 8), "hipMalloc(tables)");
         HIP_TRY(hipMemcpy(dptr, h.data(), h.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(tables)");
         NttTablesHost &T = g_tables[dev];
-        T.t14f = dptr;
-        T.t14i = dptr + nr;
+        T.tkf = dptr;
+        T.tki = dptr + nr;
         uint64_t *dt = dptr + 2 * nr;
         T.twl_f = dt; T.twh_f = dt + nt; T.twl_i = dt + 2 * nt; T.twh_i = dt + 3 * nt; T.shl = dt + 4 * nt;
         T.shh = dt + 5 * nt;
