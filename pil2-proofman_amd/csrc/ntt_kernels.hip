@@ -456,7 +456,7 @@ __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t 
 template <int R, bool INV = false>
 __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64_t *__restrict__ dst, uint32_t q,
                                                 uint32_t logb, bool first, bool last, uint32_t e1, uint32_t shift_e,
-                                                const NttTables &tb)
+                                                uint32_t l1, const NttTables &tb)
 {
     constexpr int LR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
     constexpr int G = 16 / R;
@@ -482,16 +482,23 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
             twiddle_rest<R>(v[gg], w);
         }
         dft_dit<R, INV>(v[gg]);
-        if (last) {  // position P of block m times w_NExt^(e1 * P)
-            uint64_t lo[R], hi[R];
+        if (last) {
+            // position P = (j << q) | o_lo of block m times w_NExt^(e1 * P) = D * C_j, with
+            // D = w_NExt^(e1 * o_lo) from the split tables (once per group) and
+            // C_j = w_NExt^(e1 * j * 2^q) = w_{2^(L1+LR)}^(e1 * j), one compact-table entry
+            // per j that is the same for the whole block
+            uint32_t ed = (e1 * p.o_lo) << shift_e;
+            uint64_t d = gl_mul((INV ? tb.twl_i : tb.twl_f)[ed & 4095], (INV ? tb.twh_i : tb.twh_f)[ed >> 12]);
+            uint64_t tw[R], dd[R], c[R];
+            const uint64_t *ct = (INV ? tb.tki : tb.tkf) + (1u << (l1 + LR));
 #pragma unroll
-            for (int j = 0; j < R; j++) {
-                uint32_t e = (e1 * round_m(p, j, q, LR)) << shift_e;
-                lo[j] = (INV ? tb.twl_i : tb.twl_f)[e & 4095];
-                hi[j] = (INV ? tb.twh_i : tb.twh_f)[e >> 12];
+            for (int j = 1; j < R; j++) {
+                c[j] = ct[e1 * (uint32_t)j];
+                dd[j] = d;
             }
-            gl_mul_v<R>(lo, lo, hi);
-            gl_mul_v<R>(v[gg], v[gg], lo);
+            tw[0] = d;
+            gl_mul_v<R - 1>(tw + 1, dd + 1, c + 1);
+            gl_mul_v<R>(v[gg], v[gg], tw);
 #pragma unroll
             for (int j = 0; j < R; j++) dst[round_m(p, j, q, LR)] = v[gg][j];
         } else {
@@ -538,10 +545,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
         uint32_t r = (top - qq) >= rmax ? rmax : (top - qq);
         bool last = (qq + r == top);
         switch (r) {
-        case 4: block_dit_round<16, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        case 3: block_dit_round<8, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        case 2: block_dit_round<4, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
-        default: block_dit_round<2, INV>(s, dst, qq, logb, first, last, e1, shift_e, tb); break;
+        case 4: block_dit_round<16, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        case 3: block_dit_round<8, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        case 2: block_dit_round<4, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        default: block_dit_round<2, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
         }
         if (!last) __syncthreads();
         first = false;
