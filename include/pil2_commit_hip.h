@@ -182,7 +182,9 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
  *   n_streams         graph branches, 1..16
  *   ntt_tile_log      largest NTT block tile, log2 values, 12..14 (a shape that
  *                     needs a larger tile gets it); smaller tiles fit more blocks per CU
- *   ntt_radix_log     most index bits per in-tile NTT round: 2, 3, 4 (radix 4, 8, 16)
+ *   ntt_radix_log     most index bits per in-tile round of the NTT block pass: 2, 3, 4
+ *                     (radix 4, 8, 16); the strided passes always take the fewest
+ *                     rounds of at most 4 bits
  *   coop_max_parents  tree levels with at most this many parents hash with 16 lanes
  *                     per permutation, larger levels with one lane per parent
  * The plain entry points and fields left 0 use {-, 13, 3, 16384}. */

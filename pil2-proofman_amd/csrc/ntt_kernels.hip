@@ -310,7 +310,7 @@ __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uin
 // iNTT pass 1: after the DIF the value at m is the transform at bitrev(m); it is
 // stored times w_N^-(i * bitrev_L(m)) (i = i0 + ii, N = n1 * n2).
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt, uint32_t rmax,
+                                                                 uint32_t log_n2, uint32_t lt,
                                                                  NttTables tb)
 {
     extern __shared__ uint64_t s[];
@@ -319,9 +319,12 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     uint64_t *col = data + (uint64_t)blockIdx.y * cs;
     uint32_t tile = 1u << (L + lt);
     strided_load(s, col, n2, i0, lt, tile);
-    int q = (int)L;
+    // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
+    // strided tile keeps 16 values per thread whatever the plan's radix
+    int q = (int)L, nr = ((int)L + 3) / 4, kr = 0;
     while (q > 0) {
-        int r = q >= (int)rmax ? (int)rmax : q;
+        int r = (q + (nr - kr) - 1) / (nr - kr);
+        kr++;
         q -= r;
         switch (r) {
         case 4: strided_dif_round<16>(s, lt, q, tb); break;
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
 // NTT pass 3: n1-point DIT over m (bit-reversed in, natural out).
 template <bool INV = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt, uint32_t rmax,
+                                                                 uint32_t log_n2, uint32_t lt,
                                                                  NttTables tb)
 {
     extern __shared__ uint64_t s[];
@@ -365,9 +368,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     uint64_t *col = data + (uint64_t)blockIdx.y * cs;
     uint32_t tile = 1u << (L + lt);
     strided_load(s, col, n2, i0, lt, tile);
-    uint32_t q = 0;
+    uint32_t q = 0, nr = (L + 3) / 4, kr = 0;  // rounds as in k_ntt_strided_inv
     while (q < L) {
-        uint32_t r = (L - q) >= rmax ? rmax : (L - q);
+        uint32_t r = ((L - q) + (nr - kr) - 1) / (nr - kr);
+        kr++;
         switch (r) {
         case 4: strided_dit_round<16, INV>(s, lt, q, tb); break;
         case 3: strided_dit_round<8, INV>(s, lt, q, tb); break;
@@ -844,7 +848,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
-                           L2, lt, rmax, tb);
+                           L2, lt, tb);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     // 2. fused block: DIF, coset scale, spread, DIT, twiddle
@@ -862,7 +866,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
-                           N << log_b, L1, l2b, lt, rmax, tb);
+                           N << log_b, L1, l2b, lt, tb);
     }
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
@@ -910,7 +914,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), q_dim);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, qc, NExt, L1,
-                           lb, lt, 4u, tb);
+                           lb, lt, tb);
     }
     // 2a. inverse blocks -> scaled coefficient lines (coef: column c at coef + c*N)
     hipLaunchKernelGGL(k_q_block_inv, dim3(1u << L1, q_dim), dim3(1u << (lb - 4)), lds_bytes(lb), st, qc, NExt, coef,
@@ -925,7 +929,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
-                           lb, lt, 4u, tb);
+                           lb, lt, tb);
     }
     return (int)hipGetLastError();
 }
@@ -974,10 +978,10 @@ extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, u
         dim3 gs((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         if (inverse)
             hipLaunchKernelGGL(k_ntt_strided_fwd<true>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst, M,
-                               L1, L2, lt, 4u, tb);
+                               L1, L2, lt, tb);
         else
             hipLaunchKernelGGL(k_ntt_strided_fwd<false>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst,
-                               M, L1, L2, lt, 4u, tb);
+                               M, L1, L2, lt, tb);
     }
     return (int)hipGetLastError();
 }
