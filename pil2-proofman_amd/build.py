@@ -46,7 +46,21 @@ def build(verbose=False):
             f.result()
     _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
+    build_c_host(verbose)
     return LIB
+
+
+C_HOST_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_host", "commit_host.c")
+C_HOST_BIN = os.path.join(os.path.dirname(HERE), "tests", "c_host", "commit_host")
+
+
+def build_c_host(verbose=False):
+    """A plain C program linked against the library through the public header (the
+    way a non-Python host such as proofman's Rust FFI links it); the tests run it."""
+    _run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-o", C_HOST_BIN + ".tmp", C_HOST_SRC, "-L" + HERE,
+          "-lpil2_commit_hip", "-Wl,-rpath,$ORIGIN/../../pil2-proofman_amd"], verbose)
+    os.replace(C_HOST_BIN + ".tmp", C_HOST_BIN)
+    return C_HOST_BIN
 
 
 if __name__ == "__main__":

@@ -82,3 +82,15 @@ def test_batch_shape_plan_defaults_before_tuning(lib):
     assert f(20, 22, 100, 3, ctypes.byref(p)) == 0
     assert (p.n_streams, p.ntt_tile_log, p.ntt_radix_log, p.coop_max_parents) == (0, 13, 3, 16384)
     assert f(20, 22, 100, 3, None) < 0
+
+
+def test_c_host_links_through_the_header():
+    """A plain C program built against include/pil2_commit_hip.h links with the
+    library and resolves every entry point it calls (it runs in the GPU tests)."""
+    binpath = os.path.join(ROOT, "tests", "c_host", "commit_host")
+    if not os.path.exists(binpath):
+        pytest.fail("tests/c_host/commit_host is not built; run __graft_entry__.build()")
+    out = subprocess.run(["ldd", binpath], capture_output=True, text=True).stdout
+    assert "libpil2_commit_hip.so" in out and "not found" not in out, out
+    out = subprocess.run([binpath], capture_output=True, text=True)
+    assert out.returncode == 2 and "usage" in out.stderr  # no device call without arguments
