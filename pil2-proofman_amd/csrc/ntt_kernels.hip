@@ -770,6 +770,9 @@ extern "C" int ntt_init_device(void)
     return 0;
 }
 
+// Strided-pass tile of the computeQ and plain-NTT paths (the LDE takes it from its plan).
+#define NTT_STRIDED_TILE 13
+
 static size_t lds_bytes(uint32_t log_tile) { return (size_t)8 * ((1u << log_tile) + (1u << log_tile) / 16); }
 
 // Strided-pass tile: n1 = 2^L1 points x 2^lt lines with 2^(L1 + lt) <= 2^tile values and
@@ -909,8 +912,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
     }
     // 1. strided inverse DIF over n1 (stride n2*B)
     if (L1 > 0) {
-        uint32_t lt = NTT_LOG_TILE - L1;
-        if (lt > lb) lt = lb;
+        uint32_t lt = strided_lt(L1, lb, NTT_STRIDED_TILE);
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), q_dim);
         hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, qc, NExt, L1,
@@ -924,8 +926,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
                        cmq, NExt, L1, L2, log_b, (uint64_t)0, 4u, tb);
     // 3. strided forward DIT over n1
     if (L1 > 0) {
-        uint32_t lt = NTT_LOG_TILE - L1;
-        if (lt > lb) lt = lb;
+        uint32_t lt = strided_lt(L1, lb, NTT_STRIDED_TILE);
         uint32_t log_tile = L1 + lt;
         dim3 g((unsigned)(1ull << (lb - lt)), ncols);
         hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
@@ -972,8 +973,7 @@ extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, u
         hipLaunchKernelGGL((k_lde_block<true, false>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
                            L2, 0u, (uint64_t)0, 4u, tb);
     if (L1 > 0) {
-        uint32_t lt = NTT_LOG_TILE - L1;
-        if (lt > L2) lt = L2;
+        uint32_t lt = strided_lt(L1, L2, NTT_STRIDED_TILE);
         uint32_t log_tile = L1 + lt;
         dim3 gs((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
         if (inverse)
