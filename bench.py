@@ -46,6 +46,8 @@ HBM_PEAK_GBS = 8000.0
 # VALU lane-operations per second: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
 # (= the published 157.3 TFLOP/s FP32 vector rate / 2 for FMA / 2 for packed).
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+N_SIMD = 256 * 4
+CLOCK_HZ = 2.4e9
 # Poseidon2 field multiplications per permutation (algorithmic count):
 # 8 full rounds x W x 4 (x^7) + 22 partial rounds x (4 + W)
 MULS_PER_PERM = {8: 8 * 8 * 4 + 22 * 12, 12: 8 * 12 * 4 + 22 * 16, 16: 8 * 16 * 4 + 22 * 20}
@@ -158,6 +160,17 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
     }
 
 
+def _with_issue(roofline, config):
+    """Adds the leaf kernel's measured VALU issue share to its roofline object."""
+    f = _valu_issue(roofline["kernel"], config, roofline["avg_ms"])
+    if f is not None:
+        roofline["valu_issue_frac"] = round(f, 4)
+        roofline["valu_issue_note"] = ("VALU issue slots filled (rocprof SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2 per "
+                                       "launch from profiles/, 1024 SIMDs at 2.4 GHz); the products-only 'frac' "
+                                       "counts 4 of the ~13 single-rate instructions a reduced product takes")
+    return roofline
+
+
 def run_batched(args, cfg):
     """Mixed-size batch: all instances resident in HBM, committed (a) one by one
     through the plain per-instance path (commit + root back to the host, the way
@@ -243,8 +256,9 @@ def run_batched(args, cfg):
     rate = width - 4
     perms = sum((1 << j["n_bits_ext"]) * ((j["n_cols"] + rate - 1) // rate) for j in shapes)
     leaf_bytes = sum(8 * (1 << j["n_bits_ext"]) * j["n_cols"] + 32 * (1 << j["n_bits_ext"]) for j in shapes)
-    roofline = _leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
-                              _profile_traffic("k_p2_leaves<%d>" % width, "batched"), len(shapes))
+    roofline = _with_issue(_leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
+                                          _profile_traffic("k_p2_leaves<%d>" % width, "batched"), len(shapes)),
+                           "batched")
     roofline["measured_on"] = "per-instance commits of the same batch (HIP events around each leaf launch)"
     lde_bytes = sum(8 * j["n_cols"] * ((1 << j["n_bits"]) + (1 << j["n_bits_ext"])) for j in shapes)
     lde_ms = sum(phase_ms[0:4])
@@ -676,16 +690,32 @@ def run_plain_plan(args, cfg):
     print(json.dumps(out), flush=True)
 
 
-def _profile_traffic(kernel, config):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+def _profile_kernel(kernel, config):
+    """Per-launch counters of `kernel` from the committed rocprofv3 summary, or {}."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
     try:
         with open(path) as f:
-            data = json.load(f)
-        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+            return json.load(f)["kernels"][kernel]
     except (OSError, KeyError, ValueError):
+        return {}
+
+
+def _profile_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    return _profile_kernel(kernel, config).get("hbm_bytes_per_launch")
+
+
+def _valu_issue(kernel, config, ms_per_launch):
+    """Share of the VALU issue slots (1024 SIMDs at the 2.4 GHz peak clock, one wave
+    instruction per 4 cycles, a dual-issued pair per 4 cycles) that the kernel's
+    instructions fill, from the committed SQ_INSTS_VALU / SQ_ACTIVE_INST_VALU2
+    counters per launch and the live launch time; None without a profile."""
+    c = _profile_kernel(kernel, config)
+    if "valu_wave_instr_per_launch" not in c or not ms_per_launch:
         return None
+    cycles = 4.0 * (c["valu_wave_instr_per_launch"] - c["valu_dual_issue_quad_cycles_per_launch"]) / N_SIMD
+    return cycles / (CLOCK_HZ * ms_per_launch / 1e3)
 
 
 def copy_calibration(nbytes):
@@ -784,8 +814,9 @@ def main():
     rate = width - 4
     perms = ne * ((cols + rate - 1) // rate)
     leaf_bytes = 8 * ne * cols + 32 * ne  # reads every LDE word once, writes one digest per row
-    roofline = _leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
-                              _profile_traffic("k_p2_leaves<%d>" % width, args.config), 1)
+    roofline = _with_issue(_leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
+                                          _profile_traffic("k_p2_leaves<%d>" % width, args.config), 1),
+                           args.config)
     # LDE passes: algorithmic bytes 8*C*(N + NExt) over the four LDE kernels
     lde_ms = sum(phase_ms[0:4])
     lde_bytes = 8 * cols * (n + ne)
