@@ -256,9 +256,8 @@ def run_batched(args, cfg):
     rate = width - 4
     perms = sum((1 << j["n_bits_ext"]) * ((j["n_cols"] + rate - 1) // rate) for j in shapes)
     leaf_bytes = sum(8 * (1 << j["n_bits_ext"]) * j["n_cols"] + 32 * (1 << j["n_bits_ext"]) for j in shapes)
-    roofline = _with_issue(_leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
-                                          _profile_traffic("k_p2_leaves<%d>" % width, "batched"), len(shapes)),
-                           "batched")
+    roofline = _leaf_roofline(width, perms, phase_ms[4], leaf_bytes,
+                              _profile_traffic("k_p2_leaves<%d>" % width, "batched"), len(shapes))
     roofline["measured_on"] = "per-instance commits of the same batch (HIP events around each leaf launch)"
     lde_bytes = sum(8 * j["n_cols"] * ((1 << j["n_bits"]) + (1 << j["n_bits_ext"])) for j in shapes)
     lde_ms = sum(phase_ms[0:4])
