@@ -78,17 +78,28 @@ def cpu_baseline(cfg, target_s=15.0, max_cols=None):
     try:
         lib, kind = RefLib(), "reference"
         tree_len = o.tree_num_elements(ne, arity)
-        run = lambda t: lib.merkletree(lib.lde(t, ne), arity, tree_len)
+        lde_f, tree_f = (lambda t: lib.lde(t, ne)), (lambda e: lib.merkletree(e, arity, tree_len))
         threads = lib.num_threads()
     except (OSError, FileNotFoundError):
         lib, kind = o, "port"
-        run = lambda t: o.merkletree(o.lde(t, ne), arity)
+        lde_f, tree_f = (lambda t: o.lde(t, ne)), (lambda e: o.merkletree(e, arity))
         threads = o.num_threads()
+    split = [0.0, 0.0]  # LDE seconds, merkletree seconds
+
+    def run(t):
+        t0 = time.perf_counter()
+        e = lde_f(t)
+        t1 = time.perf_counter()
+        tree_f(e)
+        split[0] += t1 - t0
+        split[1] += time.perf_counter() - t1
+
     pc_ = 4
     probe = datagen.uniform(n * pc_, cfg["seed"]).reshape(n, pc_)
     t0 = time.perf_counter()
     run(probe)
     t_probe = time.perf_counter() - t0
+    split[0] = split[1] = 0.0
     cols = int(max(1, min(max_cols or cfg["cols"], pc_ * target_s / max(t_probe, 1e-3))))
     trace = datagen.uniform(n * cols, cfg["seed"]).reshape(n, cols)
     reps, dt = 0, 0.0
@@ -98,6 +109,7 @@ def cpu_baseline(cfg, target_s=15.0, max_cols=None):
         dt += time.perf_counter() - t0
         reps += 1
     return {"value": reps * ne * cols / dt / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+            "lde_s_per_run": round(split[0] / reps, 4), "merkletree_s_per_run": round(split[1] / reps, 4),
             "sample": "2^%d x %d cols -> 2^%d rows (blowup %d), Poseidon2 W%d arity %d; %d run(s), %.2f s"
                       % (cfg["n_bits"], cols, cfg["n_bits"] + cfg["blowup_bits"], 1 << cfg["blowup_bits"],
                          4 * arity, arity, reps, dt),
@@ -118,7 +130,7 @@ def cpu_baseline_batched(shapes, arity, target_s=15.0, what="batch"):
     except (OSError, FileNotFoundError):
         lib, kind = o, "port"
         threads = o.num_threads()
-    elems, spent, used = 0, 0.0, []
+    elems, spent, used, lde_s = 0, 0.0, [], 0.0
     for j in shapes:
         n, ne, c = 1 << j["n_bits"], 1 << j["n_bits_ext"], j["n_cols"]
         if spent > 0 and spent + ne * c / (elems / spent) > target_s:
@@ -126,13 +138,19 @@ def cpu_baseline_batched(shapes, arity, target_s=15.0, what="batch"):
         trace = datagen.uniform(n * c, j["seed"]).reshape(n, c)
         t0 = time.perf_counter()
         if kind == "reference":
-            lib.merkletree(lib.lde(trace, ne), arity, o.tree_num_elements(ne, arity))
+            e = lib.lde(trace, ne)
+            t1 = time.perf_counter()
+            lib.merkletree(e, arity, o.tree_num_elements(ne, arity))
         else:
-            o.merkletree(o.lde(trace, ne), arity)
+            e = o.lde(trace, ne)
+            t1 = time.perf_counter()
+            o.merkletree(e, arity)
         spent += time.perf_counter() - t0
+        lde_s += t1 - t0
         elems += ne * c
         used.append("2^%d x %d (x%d)" % (j["n_bits"], c, ne // n))
     return {"value": elems / spent / 1e9, "unit": "GElem/s", "cores": threads, "kind": kind,
+            "lde_s": round(lde_s, 3), "merkletree_s": round(spent - lde_s, 3),
             "sample": "%d of the %s's instances [%s], LDE + Poseidon2 W%d arity %d Merkle, %.2f s"
                       % (len(used), what, ", ".join(used), 4 * arity, arity, spent),
             **host_info()}
