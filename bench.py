@@ -20,6 +20,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "pil2-proofman_amd")
 for _p in (ROOT, PKG):
@@ -824,6 +826,29 @@ def main():
     phase_ms = [ph[i] / max(ncalls, 1) for i in range(6)]
     root = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
 
+    # The same commit with the trace in pinned host memory: each step copies it
+    # to HBM, commits and reads the root back (PCIe-inclusive; `value` is not).
+    host_trace = torch.empty(n * cols, dtype=trace.dtype, pin_memory=True)
+    host_trace.copy_(trace)
+    torch.cuda.synchronize()
+    h2d_steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(h2d_steps):
+        trace.copy_(host_trace, non_blocking=True)
+        pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
+        root_h2d = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
+    h2d_dt = (time.perf_counter() - t0) / h2d_steps
+    assert np.array_equal(root_h2d, root), "commit from host memory gave another root"
+    t0 = time.perf_counter()
+    trace.copy_(host_trace, non_blocking=True)
+    torch.cuda.synchronize()
+    copy_dt = time.perf_counter() - t0
+    del host_trace
+    with_h2d = {"value": round(ne * cols / h2d_dt / 1e9, 4), "unit": "GElem/s", "ms_per_step": round(h2d_dt * 1e3, 3),
+                "steps": h2d_steps, "h2d_ms": round(copy_dt * 1e3, 3),
+                "h2d_GBs": round(8 * n * cols / copy_dt / 1e9, 2),
+                "note": "trace in pinned host memory: H2D copy + commit + root read per step"}
+
     ms_per_step = dt * 1e3 / args.steps
     value = ne * cols / (dt / args.steps) / 1e9
 
@@ -857,7 +882,7 @@ def main():
                    "path": "plain (one instance per call)"},
         "phases_ms": dict(zip(["transpose", "intt_strided", "lde_block", "ntt_strided", "leaves", "tree_levels"],
                               [round(x, 4) for x in phase_ms])),
-        "roofline": roofline, "roofline_lde": roofline_lde,
+        "roofline": roofline, "roofline_lde": roofline_lde, "with_h2d": with_h2d,
         "root": ["%016x" % v for v in root],
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
