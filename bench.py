@@ -158,6 +158,35 @@ def cpu_baseline_batched(shapes, arity, target_s=15.0, what="batch"):
             **host_info()}
 
 
+_MUL_PEAK = []
+
+
+def measured_mul_peak(lanes=1 << 21, iters=1024, reps=3):
+    """Measured throughput of the Poseidon2 kernels' own reduced product
+    (pil2hip_mul_peak: 18 independent chains per lane, the permutation's
+    p2_mul3), in T products/s, timed with HIP events on the launch stream.
+    Measured once per process, outside every timed region."""
+    if not _MUL_PEAK:
+        import torch
+        import pil2_commit as pc
+        c = pc.MUL_PEAK_CHAINS
+        inp = pc.empty(2 * c * lanes)
+        pc.fill_uniform(inp, 0x5EED00FF)
+        out = pc.empty(c * lanes)
+        pc.mul_peak(out, inp, lanes, iters)
+        st = torch.cuda.current_stream()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(st)
+        for _ in range(reps):
+            pc.mul_peak(out, inp, lanes, iters)
+        t1.record(st)
+        t1.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        _MUL_PEAK.append((lanes * iters * c / (ms / 1e3) / 1e12, ms))
+        del inp, out
+    return _MUL_PEAK[0]
+
+
 def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
     """VALU roofline of the Poseidon2 leaf kernel: algorithmic Goldilocks
     multiplications (736 per W12 permutation) against the rate at which the VALU
@@ -174,6 +203,10 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
                  "(published 157.3 TFLOP/s FP32 vector / 2 FMA / 2 packed)" % (MULS_PER_PERM[width], width,
                                                                              VALU_PEAK_TOPS),
         "perm_per_s": perms / leaf_s,
+        "mul_peak_measured": round(measured_mul_peak()[0], 4),
+        "frac_of_measured_mul_peak": round(muls_per_s / 1e12 / measured_mul_peak()[0], 4),
+        "mul_peak_note": "pil2hip_mul_peak: the kernel's own reduced product (13 single-rate instructions), "
+                         "18 independent chains per lane, 2^21 lanes",
         "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
         "traffic": traffic,
         "avg_ms": leaf_ms / launches,

@@ -385,6 +385,14 @@ int64_t pil2hip_fill_uniform(uint64_t *d_out, uint64_t n, uint64_t seed, uint64_
 int64_t pil2hip_gl_binop(uint64_t *d_out, const uint64_t *d_a, const uint64_t *d_b, uint64_t n, int32_t op,
                          void *stream);
 
+/* Throughput calibration of the Poseidon2 kernels' reduced 64x64 product (no
+ * reference counterpart; it gives the measured product peak that the leaf
+ * kernel's field multiplications per second are compared with). Lane i runs 18
+ * independent chains a[k] <- a[k]*b[k] for `iters` steps, with a[k] = d_in[36i+k],
+ * b[k] = d_in[36i+18+k], and writes d_out[18i+k] = a[k]*b[k]^iters mod p
+ * (canonical): lanes*iters*18 products. */
+int64_t pil2hip_mul_peak(uint64_t *d_out, const uint64_t *d_in, uint64_t lanes, uint32_t iters, void *stream);
+
 /* Phase timing of pil2hip_commit with HIP events on the commit's own stream (no
  * reference counterpart; the reference's TimerGPU, utils/gpu_timer.cuh, plays
  * this role). While enabled, every pil2hip_commit synchronises its stream and

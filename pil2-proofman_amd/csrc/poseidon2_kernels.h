@@ -25,6 +25,7 @@ extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs
 extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, uint64_t coop_max,
                                      hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);
+extern "C" int p2_launch_mul_peak(const uint64_t *in, uint64_t *out, uint64_t lanes, uint32_t iters, hipStream_t st);
 extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream_t st);
 extern "C" int p2_launch_transcript(uint32_t width, P2TranscriptState *t, uint64_t *io, uint64_t n, int op,
                                     hipStream_t st);

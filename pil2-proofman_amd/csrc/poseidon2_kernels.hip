@@ -514,6 +514,34 @@ __global__ void k_p2_permute_many(uint64_t *st, uint64_t count)
     for (int k = 0; k < W; k++) st[i * W + k] = gl_canon(s[k]);
 }
 
+// ---------------------------------------------------- product peak ----
+// Throughput calibration of the permutation's reduced product: lane i holds
+// MP_CHAINS independent chains a[k] <- a[k] * b[k] (the same p2_mul3 the
+// permutation runs, in groups of three) for `iters` steps; out[C*i + k] =
+// a[k] * b[k]^iters (canonical), C = MP_CHAINS. Inputs: a[k] = in[2C*i + k],
+// b[k] = in[2C*i + C + k].
+#define MP_GROUPS 6
+#define MP_CHAINS (3 * MP_GROUPS)
+__global__ __launch_bounds__(256) void k_mul_peak(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                                  uint64_t lanes, uint32_t iters)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lanes) return;
+    uint64_t a[MP_CHAINS], b[MP_CHAINS];
+#pragma unroll
+    for (int k = 0; k < MP_CHAINS; k++) {
+        a[k] = in[i * 2 * MP_CHAINS + k];
+        b[k] = in[i * 2 * MP_CHAINS + MP_CHAINS + k];
+    }
+#pragma unroll 1
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int g = 0; g < MP_GROUPS; g++) p2_mul3(a + 3 * g, b + 3 * g, a + 3 * g);
+    }
+#pragma unroll
+    for (int k = 0; k < MP_CHAINS; k++) out[i * MP_CHAINS + k] = gl_canon(a[k]);
+}
+
 // ------------------------------------------------------------ openings ----
 // Group proofs (MerkleTreeGL::getGroupProof, merkleTreeGL.cpp:145-175) for many
 // queries: block q writes proof q at out + q*pw: the nCols values of source row
@@ -648,6 +676,13 @@ extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nl
         off += (pending + pad) * 4;
         pending = next;
     }
+    return (int)hipGetLastError();
+}
+
+extern "C" int p2_launch_mul_peak(const uint64_t *in, uint64_t *out, uint64_t lanes, uint32_t iters, hipStream_t st)
+{
+    if (lanes == 0) return 0;
+    hipLaunchKernelGGL(k_mul_peak, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, in, out, lanes, iters);
     return (int)hipGetLastError();
 }
 

@@ -1110,6 +1110,15 @@ int64_t pil2hip_p2_permute(uint64_t *d_states, uint64_t count, uint32_t width, v
     return 0;
 }
 
+int64_t pil2hip_mul_peak(uint64_t *d_out, const uint64_t *d_in, uint64_t lanes, uint32_t iters, void *stream)
+{
+    if (lanes > (uint64_t)UINT32_MAX * 256) return fail(-12, "too many lanes");
+    if (lanes && (!d_out || !d_in)) return fail(-13, "null buffer");
+    int e = p2_launch_mul_peak(d_in, d_out, lanes, iters, (hipStream_t)stream);
+    if (e) return fail(-100 - e, "product calibration launch failed");
+    return 0;
+}
+
 int64_t pil2hip_gl_binop(uint64_t *d_out, const uint64_t *d_a, const uint64_t *d_b, uint64_t n, int32_t op,
                          void *stream)
 {

@@ -88,6 +88,7 @@ _SIGS = {
     "pil2hip_transpose_cm2rm": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_p2_permute": (_i64, [_vp, _u64, ctypes.c_uint32, _vp]),
     "pil2hip_fill_uniform": (_i64, [_vp, _u64, _u64, _u64, _vp]),
+    "pil2hip_mul_peak": (_i64, [_vp, _vp, _u64, ctypes.c_uint32, _vp]),
     "pil2hip_gl_binop": (_i64, [_vp, _vp, _vp, _u64, ctypes.c_int32, _vp]),
     "pil2hip_timing_enable": (None, [ctypes.c_bool]),
     "pil2hip_timing_read": (_u64, [_vp, ctypes.c_bool]),
@@ -419,6 +420,19 @@ def p2_permute(states, width, stream=None):
 def fill_uniform(out, seed, start=0, stream=None):
     """Fill a device tensor with the seeded uniform stream (datagen.uniform on device)."""
     _check(_lib.pil2hip_fill_uniform(_ptr(out), out.numel(), seed, start, _stream(stream)))
+    return out
+
+
+MUL_PEAK_CHAINS = 18
+
+
+def mul_peak(out, inp, lanes, iters, stream=None):
+    """Product throughput calibration: out[18i+k] = inp[36i+k] * inp[36i+18+k]^iters
+    mod p, as lanes*iters*18 chained products of the Poseidon2 kernels' multiply."""
+    c = MUL_PEAK_CHAINS
+    if inp.numel() < 2 * c * lanes or out.numel() < c * lanes:
+        raise ValueError("mul_peak: buffers smaller than 36*lanes / 18*lanes words")
+    _check(_lib.pil2hip_mul_peak(_ptr(out), _ptr(inp), lanes, iters, _stream(stream)))
     return out
 
 

@@ -75,6 +75,27 @@ def test_field_ops_edge_values(pc):
     assert _binop(pc, 3, lo, hi) == [((h << 64) + l) % P for l, h in zip(lo, hi)]
 
 
+def test_mul_peak_closed_form(pc):
+    """The product calibration kernel (the Poseidon2 kernels' own reduced product,
+    chained) against its closed form a * b^iters mod p, over edge words (non-canonical
+    inputs included) and random words, on a lane count that leaves a partial block."""
+    rng = np.random.default_rng(12)
+    c = pc.MUL_PEAK_CHAINS
+    lanes, iters = 1000, 37
+    words = [int(v) for v in rng.integers(0, 2**64, 2 * c * lanes, dtype=np.uint64)]
+    for k, v in enumerate(RAW_EDGE):  # edge words as both multiplicands and multipliers
+        words[2 * c * k + k % c] = v
+        words[2 * c * k + c + (k + 1) % c] = v
+    inp = pc.to_device(np.array(words, dtype=np.uint64))
+    out = pc.empty(c * lanes)
+    pc.mul_peak(out, inp, lanes, iters)
+    got = pc.to_host(out).tolist()
+    want = [words[2 * c * i + k] * pow(words[2 * c * i + c + k], iters, P) % P for i in range(lanes) for k in range(c)]
+    assert got == want
+    pc.mul_peak(out, inp, lanes, 0)  # zero steps: the canonical inputs
+    assert pc.to_host(out).tolist() == [words[2 * c * i + k] % P for i in range(lanes) for k in range(c)]
+
+
 # ------------------------------------------------------------- Poseidon2 --
 @pytest.mark.parametrize("w", [8, 12, 16])
 def test_permute_reference_goldens(pc, refvec, w):
