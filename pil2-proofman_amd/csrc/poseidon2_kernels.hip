@@ -41,59 +41,79 @@ template <> struct P2Tab<16> {
 };
 
 // External layer M_E (poseidon2_goldilocks.hpp:186-231): M4 on every 4-block,
-// then every element adds the sum of its column over the blocks. Inputs are
-// 64-bit words, outputs lazy sums (carry count < 80: M4 rows sum to at most 16,
-// at most 4 blocks plus the element itself).
+// then every element adds the sum of its column over the blocks. It runs on the
+// 32-bit halves of the input words separately, in 64-bit accumulators: the
+// output is in halves form with each accumulator below 2^39 (M4 rows sum to at
+// most 16, so a block output is below 2^36; at most 4 blocks plus the element
+// itself make a column total below 2^39).
 // M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] as additions and shifts:
 // t0 = x0+x1, t1 = x2+x3, t2 = 2x1+t1, t3 = 2x3+t0, t4 = 4t1+t3, t5 = 4t0+t2,
 // (x0, x1, x2, x3) <- (t3+t5, t5, t2+t4, t4).
-template <int W>
-__device__ __forceinline__ void ext_layer_lz(const uint64_t *s, lz *t)
+// M4 on the 32-bit halves a[0..4) and b[0..4) of two blocks, in 64-bit sums:
+// t0 and t1 are 32-bit additions with carry (no zero-extended operands), t2
+// and t3 multiply-adds with a 32-bit operand, the rest 64-bit shift-adds.
+__device__ __forceinline__ void m4_halves2(const uint32_t *a, const uint32_t *b, uint64_t *ya, uint64_t *yb)
 {
-    constexpr int NB = W / 4, N2 = 2 * NB;
-    lz x[W];
+    uint32_t l[4], h[4];
+    uint64_t c[4];
+    asm("v_add_co_u32_e64 %0, %8, %12, %13\n\t"
+        "v_add_co_u32_e64 %1, %9, %14, %15\n\t"
+        "v_add_co_u32_e64 %2, %10, %16, %17\n\t"
+        "v_add_co_u32_e64 %3, %11, %18, %19\n\t"
+        "v_addc_co_u32_e64 %4, %8, 0, 0, %8\n\t"
+        "v_addc_co_u32_e64 %5, %9, 0, 0, %9\n\t"
+        "v_addc_co_u32_e64 %6, %10, 0, 0, %10\n\t"
+        "v_addc_co_u32_e64 %7, %11, 0, 0, %11"
+        : "=&v"(l[0]), "=&v"(l[1]), "=&v"(l[2]), "=&v"(l[3]), "=&v"(h[0]), "=&v"(h[1]), "=&v"(h[2]), "=&v"(h[3]),
+          "=&s"(c[0]), "=&s"(c[1]), "=&s"(c[2]), "=&s"(c[3])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(b[0]), "v"(b[1]), "v"(b[2]), "v"(b[3]));
+    uint64_t t0a = gl_pack(l[0], h[0]), t1a = gl_pack(l[1], h[1]), t0b = gl_pack(l[2], h[2]), t1b = gl_pack(l[3], h[3]);
+    uint64_t t2a, t3a, t2b, t3b, d[4];
+    asm("v_mad_u64_u32 %0, %4, %8, 2, %12\n\t"
+        "v_mad_u64_u32 %1, %5, %9, 2, %13\n\t"
+        "v_mad_u64_u32 %2, %6, %10, 2, %14\n\t"
+        "v_mad_u64_u32 %3, %7, %11, 2, %15"
+        : "=&v"(t2a), "=&v"(t3a), "=&v"(t2b), "=&v"(t3b), "=&s"(d[0]), "=&s"(d[1]), "=&s"(d[2]), "=&s"(d[3])
+        : "v"(a[1]), "v"(a[3]), "v"(b[1]), "v"(b[3]), "v"(t1a), "v"(t0a), "v"(t1b), "v"(t0b));
+    uint64_t t4a = (t1a << 2) + t3a, t5a = (t0a << 2) + t2a, t4b = (t1b << 2) + t3b, t5b = (t0b << 2) + t2b;
+    ya[0] = t3a + t5a;
+    ya[1] = t5a;
+    ya[2] = t2a + t4a;
+    ya[3] = t4a;
+    yb[0] = t3b + t5b;
+    yb[1] = t5b;
+    yb[2] = t2b + t4b;
+    yb[3] = t4b;
+}
+
+template <int W>
+__device__ __forceinline__ void ext_layer_hz(const uint64_t *s, hz *t)
+{
+    constexpr int NB = W / 4;
+    uint64_t y[2][W], col[2][4];
+    uint32_t hv[2][W];
 #pragma unroll
-    for (int i = 0; i < W; i++) x[i] = lz_of(s[i]);
-    lz a[N2], b[N2], t01[N2], t23[N2], t45[N2], t67[N2];
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        a[2 * j] = x[4 * j]; b[2 * j] = x[4 * j + 1];
-        a[2 * j + 1] = x[4 * j + 2]; b[2 * j + 1] = x[4 * j + 3];
+    for (int i = 0; i < W; i++) {
+        hv[0][i] = (uint32_t)s[i];
+        hv[1][i] = (uint32_t)(s[i] >> 32);
     }
-    lz_add_all<N2>(t01, a, b);  // t0, t1
+    // the 2*NB half-blocks, two at a time
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-        a[2 * j] = lz_shl<1>(x[4 * j + 1]); b[2 * j] = t01[2 * j + 1];
-        a[2 * j + 1] = lz_shl<1>(x[4 * j + 3]); b[2 * j + 1] = t01[2 * j];
+    for (int k = 0; k < NB; k++) m4_halves2(&hv[0][4 * k], &hv[1][4 * k], &y[0][4 * k], &y[1][4 * k]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            col[h][k] = y[h][k];
+#pragma unroll
+            for (int j = 1; j < NB; j++) col[h][k] += y[h][4 * j + k];
+        }
     }
-    lz_add_all<N2>(t23, a, b);  // t2, t3
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-        a[2 * j] = lz_shl<2>(t01[2 * j + 1]); b[2 * j] = t23[2 * j + 1];
-        a[2 * j + 1] = lz_shl<2>(t01[2 * j]); b[2 * j + 1] = t23[2 * j];
+    for (int i = 0; i < W; i++) {
+        t[i].lo = y[0][i] + col[0][i & 3];
+        t[i].hi = y[1][i] + col[1][i & 3];
     }
-    lz_add_all<N2>(t45, a, b);  // t4, t5
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        a[2 * j] = t23[2 * j + 1]; b[2 * j] = t45[2 * j + 1];
-        a[2 * j + 1] = t23[2 * j]; b[2 * j + 1] = t45[2 * j];
-    }
-    lz_add_all<N2>(t67, a, b);  // t6, t7
-    lz y[W];
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        y[4 * j] = t67[2 * j]; y[4 * j + 1] = t45[2 * j + 1];
-        y[4 * j + 2] = t67[2 * j + 1]; y[4 * j + 3] = t45[2 * j];
-    }
-    // column sums over the blocks
-    lz col[4];
-    lz_add_all<4>(col, y, y + 4);
-#pragma unroll
-    for (int j = 2; j < NB; j++) lz_add_all<4>(col, col, y + 4 * j);
-    lz cc[W];
-#pragma unroll
-    for (int i = 0; i < W; i++) cc[i] = col[i & 3];
-    lz_add_all<W>(t, y, cc);
 }
 
 // x^7 (any 64-bit word in, congruent word out)
@@ -117,38 +137,34 @@ __device__ __forceinline__ void pow7_3(const uint64_t *x, uint64_t *out)
 
 // s[i] = fold(t[i]) for all lanes, three at a time
 template <int W>
-__device__ __forceinline__ void fold_all(const lz *t, uint64_t *s)
+__device__ __forceinline__ void fold_all(const hz *t, uint64_t *s)
 {
     constexpr int G = W / 3 * 3;
 #pragma unroll
-    for (int i = 0; i < G; i += 3) lz_fold3(t + i, s + i);
+    for (int i = 0; i < G; i += 3) hz_fold3(t + i, s + i);
 #pragma unroll
-    for (int i = G; i < W; i++) s[i] = lz_fold(t[i]);
+    for (int i = G; i < W; i++) s[i] = hz_fold(t[i]);
 }
 
-// Full round on a lazy state t: S-box(t + rc) on every element, then M_E.
+// Full round on a halves-form state t: S-box(t + rc) on every element, then M_E.
+// The round constant's halves go into the accumulators (still below 2^40).
 template <int W>
-__device__ __forceinline__ void full_round(lz *t, int rc_base)
+__device__ __forceinline__ void full_round(hz *t, int rc_base)
 {
     constexpr int G = W / 3 * 3;  // lanes handled three at a time
-    uint64_t rcv[W], x[W], s[W];
-    lz u[W];
+    uint64_t x[W], s[W];
 #pragma unroll
-    for (int i = 0; i < W; i++) rcv[i] = P2Tab<W>::rc(rc_base + i);
-#pragma unroll
-    for (int i = 0; i < G; i += 3) lz_add_rc3(u + i, t + i, rcv + i);
-    if constexpr (G < W) {
-        lz rc[W - G];
-#pragma unroll
-        for (int i = 0; i < W - G; i++) rc[i] = lz_of(rcv[G + i]);
-        lz_add_all<W - G>(u + G, t + G, rc);
+    for (int i = 0; i < W; i++) {
+        uint64_t rc = P2Tab<W>::rc(rc_base + i);
+        t[i].lo += (uint32_t)rc;
+        t[i].hi += rc >> 32;
     }
-    fold_all<W>(u, x);
+    fold_all<W>(t, x);
 #pragma unroll
     for (int i = 0; i < G; i += 3) pow7_3(x + i, s + i);
 #pragma unroll
     for (int i = G; i < W; i++) s[i] = pow7(x[i]);
-    ext_layer_lz<W>(s, t);
+    ext_layer_hz<W>(s, t);
 }
 
 // Sum of N lazy values (pairwise tree, additions interleaved per level).
@@ -181,8 +197,8 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
     // Round loops stay rolled: the unrolled permutation would not fit the
     // instruction cache, and per-round constants are wave-uniform scalar loads.
     constexpr int G = W / 3 * 3;
-    lz t[W];
-    ext_layer_lz<W>(s, t);
+    hz t[W];
+    ext_layer_hz<W>(s, t);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) full_round<W>(t, r * W);
     fold_all<W>(t, s);
@@ -203,7 +219,7 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
         for (int i = G; i < W; i++) s[i] = p2_mul_add(s[i], d[i], s_lo, s_hi);
     }
 #pragma unroll
-    for (int i = 0; i < W; i++) t[i] = lz_of(s[i]);
+    for (int i = 0; i < W; i++) t[i] = {(uint32_t)s[i], s[i] >> 32};
 #pragma unroll 1
     for (int r = 0; r < 4; r++) full_round<W>(t, 4 * W + 22 + r * W);
     fold_all<W>(t, s);
