@@ -191,6 +191,29 @@ __device__ __forceinline__ lz sum_lz(const uint64_t *s)
     return sum_tree<W>(v);
 }
 
+// Sum of W words in halves form: the low and the high 32-bit halves are summed
+// separately, each as one 32-bit addition with carry and then a chain of
+// v_mad_u64_u32 (half * 1 + accumulator), which takes the 32-bit half as it is
+// (no zero-extended copy). Each accumulator stays below W * 2^32.
+template <int W>
+__device__ __forceinline__ hz sum_halves(const uint64_t *s)
+{
+    hz a;
+    a.lo = (uint64_t)(uint32_t)s[0] + (uint32_t)s[1];
+    a.hi = (s[0] >> 32) + (s[1] >> 32);
+#pragma unroll
+    for (int i = 2; i < W; i++) {
+        uint64_t nl, nh, d0, d1;
+        asm("v_mad_u64_u32 %0, %2, %4, 1, %6\n\t"
+            "v_mad_u64_u32 %1, %3, %5, 1, %7"
+            : "=&v"(nl), "=&v"(nh), "=&s"(d0), "=&s"(d1)
+            : "v"((uint32_t)s[i]), "v"((uint32_t)(s[i] >> 32)), "v"(a.lo), "v"(a.hi));
+        a.lo = nl;
+        a.hi = nh;
+    }
+    return a;
+}
+
 template <int W>
 __device__ __forceinline__ void p2_permute(uint64_t *s)
 {
@@ -208,7 +231,7 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
         lz rc0 = lz_of(P2Tab<W>::rc(4 * W + r)), u, x0 = lz_of(s[0]);
         lz_add_n<1>(&u, &x0, &rc0);
         s[0] = pow7(lz_fold(u));
-        uint64_t sum = lz_fold(sum_lz<W>(s));
+        uint64_t sum = hz_fold(sum_halves<W>(s));
         uint32_t s_lo = (uint32_t)sum, s_hi = (uint32_t)(sum >> 32);
         uint64_t d[W];
 #pragma unroll
