@@ -337,6 +337,20 @@ __device__ __forceinline__ lz dpp_lz(lz v)
     return r;
 }
 
+// the same barrier for two 64-bit values
+__device__ __forceinline__ void dpp_ready64(uint64_t &a, uint64_t &b)
+{
+    asm volatile("s_nop 1" : "+v"(a), "+v"(b));
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return gl_pack(lo, hi);
+}
+
 constexpr int dpp_quad_bcast(int k) { return k | (k << 2) | (k << 4) | (k << 6); }
 constexpr int dpp_row_ror(int n) { return 0x120 + n; }
 
@@ -386,18 +400,20 @@ __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t s, uint32_t i)
     for (int r = 0; r < 22; r++) {
         uint64_t x0 = pow7(lz_fold(lz_add1(lz_of(s), lz_of(P2Tab<W>::rc(4 * W + r)))));
         s = i == 0 ? x0 : s;
-        lz v = lz_of(s);
+        // row sum in halves form: the two 32-bit halves in 64-bit accumulators
+        // (below 16 * 2^32 after the four rotations), one 64-bit add per step
+        hz v = {(uint32_t)s, s >> 32};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            dpp_ready(v.l0, v.l1, v.c);
+            dpp_ready64(v.lo, v.hi);
             switch (k) {
-            case 0: v = lz_add1(v, dpp_lz<dpp_row_ror(1)>(v)); break;
-            case 1: v = lz_add1(v, dpp_lz<dpp_row_ror(2)>(v)); break;
-            case 2: v = lz_add1(v, dpp_lz<dpp_row_ror(4)>(v)); break;
-            default: v = lz_add1(v, dpp_lz<dpp_row_ror(8)>(v)); break;
+            case 0: v.lo += dpp_u64<dpp_row_ror(1)>(v.lo); v.hi += dpp_u64<dpp_row_ror(1)>(v.hi); break;
+            case 1: v.lo += dpp_u64<dpp_row_ror(2)>(v.lo); v.hi += dpp_u64<dpp_row_ror(2)>(v.hi); break;
+            case 2: v.lo += dpp_u64<dpp_row_ror(4)>(v.lo); v.hi += dpp_u64<dpp_row_ror(4)>(v.hi); break;
+            default: v.lo += dpp_u64<dpp_row_ror(8)>(v.lo); v.hi += dpp_u64<dpp_row_ror(8)>(v.hi); break;
             }
         }
-        uint64_t sum = lz_fold(v);
+        uint64_t sum = hz_fold(v);
         s = live ? p2_mul_add(s, d, (uint32_t)sum, (uint32_t)(sum >> 32)) : 0;
     }
     t = lz_of(s);
