@@ -120,6 +120,26 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
 
+/* pil2hip_commit for a trace in host memory, with the transfer overlapped (no
+ * reference counterpart with this signature: it is the column-chunk absorb of the
+ * reference's streaming commit, stream_commit.cuh:9-20 / commit_witness_streaming,
+ * starks_api.hpp:216, for an unpacked trace and Poseidon2). The row-major host
+ * trace (2^nBits x nCols) is copied in groups of groupCols columns (a multiple of
+ * the sponge rate 4*arity-4; 0 picks one group under 256 MiB of trace, else about
+ * eight groups of at least 16 columns) on a copy stream of the
+ * library's own, double-buffered, while `stream` extends the previous group into
+ * its columns of d_lde and absorbs it into the row sponges, whose capacity words
+ * carry over in the leaf digests of d_nodes. d_lde and d_nodes end up equal to
+ * pil2hip_commit's. d_stage: pil2hip_commit_host_stage_words() words. h_trace
+ * should be pinned (hipHostMalloc) for the copies to run concurrently. Returns
+ * once every copy has completed (h_trace and d_stage are then free); the LDE
+ * and tree complete on `stream`. -33: groupCols not a multiple of the rate. */
+int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *h_trace, uint64_t *d_stage,
+                            uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t groupCols,
+                            void *stream);
+/* Staging words pil2hip_commit_host needs: 3 * 2^nBits * (group width); 0 for a bad arity. */
+uint64_t pil2hip_commit_host_stage_words(uint64_t nBits, uint64_t nCols, uint64_t arity, uint64_t groupCols);
+
 /* NTT_Goldilocks::NTT / ::INTT (ntt_goldilocks.hpp:182-191, ntt_goldilocks.cpp:211-260):
  * per column, out_k = sum_j x_j w^(j*k) with w = W[nBits] (inverse: w^-1 and a
  * factor 2^-nBits), natural order in and out. rowMajor: element (r, c) at

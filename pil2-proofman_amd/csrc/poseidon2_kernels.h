@@ -22,6 +22,9 @@ struct P2TranscriptState {
 
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st);
+// carry != 0: continue the row sponges from the capacity words already in out
+extern "C" int p2_launch_leaves_carry(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
+                                      uint64_t nrows, uint64_t *out, uint32_t carry, hipStream_t st);
 extern "C" int p2_launch_tree_levels(uint32_t width, uint64_t *tree, uint64_t nleaves, uint64_t coop_max,
                                      hipStream_t st);
 extern "C" int p2_launch_permute(uint32_t width, uint64_t *states, uint64_t count, hipStream_t st);

@@ -250,9 +250,15 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
 
 // ------------------------------------------------------------ leaves ----
 // Row r of the source: element c at src[r*rs + c*cs]. Digest to out[r*4 .. r*4+3].
+// carry != 0 continues a sponge over later columns: the capacity starts from
+// the digest words already in out (the state words 0..3 after the previous
+// columns, which is what the next block's capacity takes), so absorbing a row's
+// columns in groups whose widths are multiples of the rate gives the one-call
+// digest.
 template <int W>
 __global__ __launch_bounds__(P2_THREADS) void k_p2_leaves(const uint64_t *__restrict__ src, uint64_t rs, uint64_t cs,
-                                                          uint64_t ncols, uint64_t nrows, uint64_t *__restrict__ out)
+                                                          uint64_t ncols, uint64_t nrows, uint64_t *__restrict__ out,
+                                                          uint32_t carry)
 {
     constexpr int RATE = W - 4;
     uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -260,10 +266,12 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_leaves(const uint64_t *__rest
     uint64_t s[W];
     const uint64_t *row = src + r * rs;
 #pragma unroll
-    for (int i = RATE; i < W; i++) s[i] = 0;
+    for (int i = RATE; i < W; i++) s[i] = carry ? out[r * 4 + (i - RATE)] : 0;
     if (ncols == 0) {
+        if (!carry) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) out[r * 4 + i] = 0;
+            for (int i = 0; i < 4; i++) out[r * 4 + i] = 0;
+        }
         return;
     }
     for (uint64_t c0 = 0; c0 < ncols; c0 += RATE) {
@@ -449,21 +457,23 @@ __global__ __launch_bounds__(P2_THREADS) void k_p2_nodes_coop(uint64_t *__restri
 template <int W>
 __global__ __launch_bounds__(P2_THREADS) void k_p2_leaves_coop(const uint64_t *__restrict__ src, uint64_t rs,
                                                                uint64_t cs, uint64_t ncols, uint64_t nrows,
-                                                               uint64_t *__restrict__ out)
+                                                               uint64_t *__restrict__ out, uint32_t carry)
 {
     constexpr int RATE = W - 4;
     uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     uint32_t i = threadIdx.x & 15;
     bool active = r < nrows;  // whole rows are active or not
     if (ncols == 0) {
-        if (active && i < 4) out[r * 4 + i] = 0;
+        if (active && i < 4 && !carry) out[r * 4 + i] = 0;
         return;
     }
     const uint64_t *row = src + (active ? r : 0) * rs;
     uint64_t s = 0;
     for (uint64_t c0 = 0; c0 < ncols; c0 += RATE) {
         uint64_t carried = 0;
-        if (c0 > 0) {  // capacity <- previous output words 0..3
+        if (c0 == 0 && carry && active && i >= (uint32_t)RATE && i < (uint32_t)W) {
+            carried = out[r * 4 + (i - RATE)];  // continued sponge (see k_p2_leaves)
+        } else if (c0 > 0) {  // capacity <- previous output words 0..3
             uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32), z = 0;
             dpp_ready(lo, hi, z);
             lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, dpp_row_ror(RATE), 0xF, 0xF, false);
@@ -683,22 +693,28 @@ extern "C" int p2_launch_copy_root(const uint64_t *src, uint64_t *dst, hipStream
 extern "C" int p2_launch_leaves(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
                                 uint64_t nrows, uint64_t *out, hipStream_t st)
 {
+    return p2_launch_leaves_carry(width, src, rs, cs, ncols, nrows, out, 0, st);
+}
+
+extern "C" int p2_launch_leaves_carry(uint32_t width, const uint64_t *src, uint64_t rs, uint64_t cs, uint64_t ncols,
+                                      uint64_t nrows, uint64_t *out, uint32_t carry, hipStream_t st)
+{
     if (nrows == 0) return 0;
     if (nrows <= P2_COOP_LEAF_MAX_ROWS) {  // too few rows to fill the GPU: 16 lanes per row
         dim3 g((unsigned)((nrows * 16 + P2_THREADS - 1) / P2_THREADS));
         switch (width) {
-        case 8: hipLaunchKernelGGL(k_p2_leaves_coop<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
-        case 12: hipLaunchKernelGGL(k_p2_leaves_coop<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
-        case 16: hipLaunchKernelGGL(k_p2_leaves_coop<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+        case 8: hipLaunchKernelGGL(k_p2_leaves_coop<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
+        case 12: hipLaunchKernelGGL(k_p2_leaves_coop<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
+        case 16: hipLaunchKernelGGL(k_p2_leaves_coop<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
         default: return -1;
         }
         return (int)hipGetLastError();
     }
     dim3 g((unsigned)((nrows + P2_THREADS - 1) / P2_THREADS));
     switch (width) {
-    case 8: hipLaunchKernelGGL(k_p2_leaves<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
-    case 12: hipLaunchKernelGGL(k_p2_leaves<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
-    case 16: hipLaunchKernelGGL(k_p2_leaves<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out); break;
+    case 8: hipLaunchKernelGGL(k_p2_leaves<8>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
+    case 12: hipLaunchKernelGGL(k_p2_leaves<12>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
+    case 16: hipLaunchKernelGGL(k_p2_leaves<16>, g, dim3(P2_THREADS), 0, st, src, rs, cs, ncols, nrows, out, carry); break;
     default: return -1;
     }
     return (int)hipGetLastError();

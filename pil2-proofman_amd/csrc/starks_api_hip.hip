@@ -302,6 +302,97 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
                        timing_events());
 }
 
+// ------------------------------------------------------- streamed commit --
+// A host-resident trace committed with its transfer overlapped: the column-chunk
+// absorb of the reference's streaming commit (stream_commit.cuh:9-20), here for
+// an unpacked trace and Poseidon2. Columns go in groups of w (a whole number of
+// rate blocks, except the last group). A copy stream moves group g out of the
+// host rows (hipMemcpy2DAsync) into one of two row-major staging buffers while
+// the compute stream extends group g-1 into its columns of the LDE and absorbs
+// them into the row sponges, whose capacity carries over between groups in the
+// leaf-digest words of d_nodes. The tree levels follow the last group.
+// Automatic width: one group for traces under 256 MiB (the copy is short and
+// per-group launches would cost more than it hides); otherwise about eight
+// groups of whole rate blocks, at least 16 columns wide: host rows narrower than
+// 128 B made the 2D copy about half as fast (2^22 x 128 at blowup 4 from pinned
+// memory: 8 columns 231.6 ms, 16 columns 150.5 ms, 32 columns 152.1 ms, 64
+// columns 158.1 ms).
+static uint64_t stream_group_cols(uint64_t nBits, uint64_t nCols, uint64_t arity, uint64_t groupCols)
+{
+    uint64_t all = std::max<uint64_t>(nCols, 1);
+    if (groupCols) return std::min(groupCols, all);
+    if (nBits >= 40 || (nCols << nBits) * 8 < (256ull << 20)) return all;
+    uint64_t rate = 4 * arity - 4, blocks = (nCols + rate - 1) / rate;
+    uint64_t w = std::max(rate * ((16 + rate - 1) / rate), rate * ((blocks + 7) / 8));
+    return std::min(w, all);
+}
+
+uint64_t pil2hip_commit_host_stage_words(uint64_t nBits, uint64_t nCols, uint64_t arity, uint64_t groupCols)
+{
+    if (arity < 2 || arity > 4 || nBits > 40) return 0;
+    return 3 * (1ull << nBits) * stream_group_cols(nBits, nCols, arity, groupCols);
+}
+
+int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *h_trace, uint64_t *d_stage,
+                            uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t groupCols,
+                            void *stream)
+{
+    int64_t rc = check_family();
+    if (rc) return rc;
+    if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
+    if (!d_nodes || (nCols && (!d_lde || !h_trace || !d_stage))) return fail(-13, "null buffer");
+    uint64_t rate = 4 * arity - 4, w = stream_group_cols(nBits, nCols, arity, groupCols);
+    if (w < nCols && w % rate) return fail(-33, "groupCols must be a multiple of the sponge rate 4*arity-4");
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
+    uint32_t width = (uint32_t)(4 * arity);
+    int e = 0;
+    if (nCols == 0) {  // empty rows: zero digests
+        e = p2_launch_leaves(width, d_lde, 1, NExt, 0, NExt, d_nodes, st);
+    } else {
+        hipStream_t cs = nullptr;
+        hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+        auto done = [&](int64_t r) {
+            if (cs) (void)hipStreamSynchronize(cs);  // the host rows and staging are free after this
+            for (int k = 0; k < 2; k++) {
+                if (copied[k]) (void)hipEventDestroy(copied[k]);
+                if (freed[k]) (void)hipEventDestroy(freed[k]);
+            }
+            if (cs) (void)hipStreamDestroy(cs);
+            return r;
+        };
+        hipError_t he = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+        for (int k = 0; k < 2 && he == hipSuccess; k++) {
+            he = hipEventCreateWithFlags(&copied[k], hipEventDisableTiming);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&freed[k], hipEventDisableTiming);
+        }
+        if (he != hipSuccess) return done(fail(-100 - (int64_t)he, "streamed commit: stream/event creation failed"));
+        uint64_t *scratch = d_stage + 2 * N * w;
+        for (uint64_t g = 0, c0 = 0; c0 < nCols; g++, c0 += w) {
+            uint64_t wg = std::min(w, nCols - c0), *buf = d_stage + (g & 1) * N * w;
+            if (g >= 2) he = hipStreamWaitEvent(cs, freed[g & 1], 0);  // group g-2's LDE has read buf
+            if (he == hipSuccess)
+                he = hipMemcpy2DAsync(buf, wg * 8, h_trace + c0, nCols * 8, wg * 8, N, hipMemcpyHostToDevice, cs);
+            if (he == hipSuccess) he = hipEventRecord(copied[g & 1], cs);
+            if (he == hipSuccess) he = hipStreamWaitEvent(st, copied[g & 1], 0);
+            if (he != hipSuccess) return done(fail(-100 - (int64_t)he, "streamed commit: copy failed"));
+            if ((rc = lde_impl(d_lde + c0 * NExt, buf, scratch, nBits, nBitsExt, wg, st, nullptr))) return done(rc);
+            if ((he = hipEventRecord(freed[g & 1], st)) != hipSuccess)
+                return done(fail(-100 - (int64_t)he, "streamed commit: event record failed"));
+            e = p2_launch_leaves_carry(width, d_lde + c0 * NExt, 1, NExt, wg, NExt, d_nodes, g > 0, st);
+            if (e) return done(fail(-100 - e, "leaf hashing launch failed"));
+        }
+        done(0);
+    }
+    if (e) return fail(-100 - e, "leaf hashing launch failed");
+    e = p2_launch_tree_levels(width, d_nodes, NExt, P2_COOP_MAX_PARENTS, st);
+    if (e) return fail(-100 - e, "tree level launch failed");
+    return 0;
+}
+
 int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch, uint64_t nBits, uint64_t nCols,
                     bool inverse, bool rowMajor, void *stream)
 {

@@ -89,6 +89,8 @@ _SIGS = {
     "pil2hip_p2_permute": (_i64, [_vp, _u64, ctypes.c_uint32, _vp]),
     "pil2hip_fill_uniform": (_i64, [_vp, _u64, _u64, _u64, _vp]),
     "pil2hip_mul_peak": (_i64, [_vp, _vp, _u64, ctypes.c_uint32, _vp]),
+    "pil2hip_commit_host": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_commit_host_stage_words": (_u64, [_u64, _u64, _u64, _u64]),
     "pil2hip_gl_binop": (_i64, [_vp, _vp, _vp, _u64, ctypes.c_int32, _vp]),
     "pil2hip_timing_enable": (None, [ctypes.c_bool]),
     "pil2hip_timing_read": (_u64, [_vp, ctypes.c_bool]),
@@ -424,6 +426,33 @@ def fill_uniform(out, seed, start=0, stream=None):
 
 
 MUL_PEAK_CHAINS = 18
+
+
+def commit_host(h_trace, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, stage=None, group_cols=0,
+                stream=None):
+    """pil2hip_commit for a host trace (a pinned CPU tensor, or a numpy uint64
+    array), with the column groups' transfer overlapped. Returns (lde, nodes)."""
+    torch = _torch()
+    ne = 1 << n_bits_ext
+    if isinstance(h_trace, np.ndarray):
+        h = np.ascontiguousarray(h_trace, dtype=np.uint64).reshape(-1)
+        ptr, numel = h.ctypes.data, h.size
+    else:
+        if h_trace.is_cuda or not h_trace.is_contiguous():
+            raise ValueError("commit_host: h_trace must be a contiguous host tensor")
+        ptr, numel = h_trace.data_ptr(), h_trace.numel()
+    if numel < (1 << n_bits) * n_cols:
+        raise ValueError("commit_host: trace smaller than 2^n_bits * n_cols")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    lde = empty(max(ne * n_cols, 1), dev) if lde is None else lde
+    nodes = empty(get_tree_num_elements(ne, arity), dev) if nodes is None else nodes
+    words = _lib.pil2hip_commit_host_stage_words(n_bits, n_cols, arity, group_cols)
+    stage = empty(max(words, 1), dev) if stage is None else stage
+    if stage.numel() < words:
+        raise ValueError("commit_host: stage smaller than %d words" % words)
+    _check(_lib.pil2hip_commit_host(_ptr(lde), _ptr(nodes), ctypes.c_void_p(ptr), _ptr(stage), arity, n_bits,
+                                    n_bits_ext, n_cols, group_cols, _stream(stream)))
+    return lde, nodes
 
 
 def mul_peak(out, inp, lanes, iters, stream=None):

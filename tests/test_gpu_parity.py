@@ -214,6 +214,45 @@ def test_commit_is_deterministic(pc):
     assert all((r == roots[0]).all() for r in roots)
 
 
+@pytest.mark.parametrize("nb,nbe,c,arity,gc", [
+    (10, 12, 13, 3, 0),    # automatic groups, ragged last group
+    (10, 11, 40, 3, 8),    # five full groups of one rate block
+    (12, 14, 40, 3, 16),   # 16 + 16 + 8
+    (9, 11, 7, 2, 4),      # width 8, rate 4
+    (11, 12, 33, 4, 12),   # width 16, rate 12, ragged
+    (10, 12, 8, 3, 8),     # one group
+    (13, 15, 24, 3, 8),    # 2^15 rows: one-lane-per-row leaf kernel
+    (8, 9, 0, 3, 0),       # no columns: zero digests
+    (20, 21, 64, 3, 0),    # 512 MiB of trace: automatic width, four groups of 16
+])
+def test_commit_host_matches_device_commit(pc, nb, nbe, c, arity, gc):
+    """The streamed host commit (column groups copied on their own stream and
+    absorbed with the capacity carried between groups) gives pil2hip_commit's LDE
+    and tree, from pinned and from pageable host memory, on repeated calls."""
+    import torch
+    print("commit_host case", nb, nbe, c, arity, gc, flush=True)
+    n, ne = 1 << nb, 1 << nbe
+    d = pc.empty(max(n * c, 1))
+    pc.fill_uniform(d, 0xC0FFEE + c)
+    lde_ref, nodes_ref = pc.commit(d, arity, nb, nbe, c)
+    torch.cuda.synchronize()
+    want_lde, want_nodes = pc.to_host(lde_ref)[:ne * c], pc.to_host(nodes_ref)
+    pinned = torch.empty(max(n * c, 1), dtype=torch.int64, pin_memory=True)
+    pinned.copy_(d.cpu())
+    for src in (pinned, pc.to_host(d)[:n * c], pinned):
+        lde, nodes = pc.commit_host(src, arity, nb, nbe, c, group_cols=gc)
+        torch.cuda.synchronize()
+        assert np.array_equal(pc.to_host(nodes), want_nodes)
+        assert np.array_equal(pc.to_host(lde)[:ne * c], want_lde)
+
+
+def test_commit_host_rejects_group_width_off_the_rate(pc):
+    import torch
+    h = torch.zeros(1 << 10, dtype=torch.int64, pin_memory=True)
+    with pytest.raises(pc.HipError, match="multiple of the sponge rate"):
+        pc.commit_host(h, 3, 6, 7, 16, group_cols=5)
+
+
 def test_fill_uniform_matches_datagen(pc):
     d = pc.empty(100003)
     pc.fill_uniform(d, 0x5EED0002, start=11)
