@@ -357,6 +357,7 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
         hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
         auto done = [&](int64_t r) {
             if (cs) (void)hipStreamSynchronize(cs);  // the host rows and staging are free after this
+            if (r) (void)hipStreamSynchronize(st);   // on an error nothing queued may still read d_stage
             for (int k = 0; k < 2; k++) {
                 if (copied[k]) (void)hipEventDestroy(copied[k]);
                 if (freed[k]) (void)hipEventDestroy(freed[k]);
