@@ -742,6 +742,49 @@ def run_plain_plan(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def _plain_from_host(args, pc, trace, lde, nodes, scratch, root, nb, nbe, cols, arity):
+    """The plain commit with the trace in pinned host memory: copy-then-commit
+    and the streamed pil2hip_commit_host, each step ending with the root on the
+    host (PCIe-inclusive figures; `value` is the HBM-resident rate)."""
+    import torch
+    n, ne, dev = 1 << nb, 1 << nbe, trace.device
+    host_trace = torch.empty(n * cols, dtype=trace.dtype, pin_memory=True)
+    host_trace.copy_(trace)
+    torch.cuda.synchronize()
+    h2d_steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(h2d_steps):
+        trace.copy_(host_trace, non_blocking=True)
+        pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
+        root_h2d = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
+    h2d_dt = (time.perf_counter() - t0) / h2d_steps
+    assert np.array_equal(root_h2d, root), "commit from host memory gave another root"
+    t0 = time.perf_counter()
+    trace.copy_(host_trace, non_blocking=True)
+    torch.cuda.synchronize()
+    copy_dt = time.perf_counter() - t0
+    # the same from host memory through pil2hip_commit_host: column groups copied
+    # on the library's copy stream while the previous group is extended and hashed
+    stage = pc.empty(pc.lib.pil2hip_commit_host_stage_words(nb, cols, arity, 0), dev)
+    pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(h2d_steps):
+        pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
+        root_s = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
+    streamed_dt = (time.perf_counter() - t0) / h2d_steps
+    assert np.array_equal(root_s, root), "streamed commit from host memory gave another root"
+    del host_trace, stage
+    with_h2d = {"value": round(ne * cols / h2d_dt / 1e9, 4), "unit": "GElem/s", "ms_per_step": round(h2d_dt * 1e3, 3),
+                "steps": h2d_steps, "h2d_ms": round(copy_dt * 1e3, 3),
+                "h2d_GBs": round(8 * n * cols / copy_dt / 1e9, 2),
+                "note": "trace in pinned host memory: H2D copy + commit + root read per step",
+                "streamed": {"value": round(ne * cols / streamed_dt / 1e9, 4), "ms_per_step": round(streamed_dt * 1e3, 3),
+                             "note": "pil2hip_commit_host: column groups copied on a copy stream, overlapped with "
+                                     "the previous group's LDE and leaf absorb"}}
+    return with_h2d
+
+
 def _profile_kernel(kernel, config):
     """Per-launch counters of `kernel` from the committed rocprofv3 summary, or {}."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
@@ -791,6 +834,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="plain_large", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-h2d", action="store_true",
+                    help="plain configs: skip the passes from host memory (the with_h2d object)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate duration of the bounded CPU-baseline sample")
     ap.add_argument("--batch-streams", type=int, default=0,
@@ -861,40 +906,9 @@ def main():
 
     # The same commit with the trace in pinned host memory: each step copies it
     # to HBM, commits and reads the root back (PCIe-inclusive; `value` is not).
-    host_trace = torch.empty(n * cols, dtype=trace.dtype, pin_memory=True)
-    host_trace.copy_(trace)
-    torch.cuda.synchronize()
-    h2d_steps = max(1, min(args.steps, 3))
-    t0 = time.perf_counter()
-    for _ in range(h2d_steps):
-        trace.copy_(host_trace, non_blocking=True)
-        pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
-        root_h2d = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
-    h2d_dt = (time.perf_counter() - t0) / h2d_steps
-    assert np.array_equal(root_h2d, root), "commit from host memory gave another root"
-    t0 = time.perf_counter()
-    trace.copy_(host_trace, non_blocking=True)
-    torch.cuda.synchronize()
-    copy_dt = time.perf_counter() - t0
-    # the same from host memory through pil2hip_commit_host: column groups copied
-    # on the library's copy stream while the previous group is extended and hashed
-    stage = pc.empty(pc.lib.pil2hip_commit_host_stage_words(nb, cols, arity, 0), dev)
-    pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(h2d_steps):
-        pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
-        root_s = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
-    streamed_dt = (time.perf_counter() - t0) / h2d_steps
-    assert np.array_equal(root_s, root), "streamed commit from host memory gave another root"
-    del host_trace, stage
-    with_h2d = {"value": round(ne * cols / h2d_dt / 1e9, 4), "unit": "GElem/s", "ms_per_step": round(h2d_dt * 1e3, 3),
-                "steps": h2d_steps, "h2d_ms": round(copy_dt * 1e3, 3),
-                "h2d_GBs": round(8 * n * cols / copy_dt / 1e9, 2),
-                "note": "trace in pinned host memory: H2D copy + commit + root read per step",
-                "streamed": {"value": round(ne * cols / streamed_dt / 1e9, 4), "ms_per_step": round(streamed_dt * 1e3, 3),
-                             "note": "pil2hip_commit_host: column groups copied on a copy stream, overlapped with "
-                                     "the previous group's LDE and leaf absorb"}}
+    with_h2d = None
+    if not args.no_h2d:
+        with_h2d = _plain_from_host(args, pc, trace, lde, nodes, scratch, root, nb, nbe, cols, arity)
 
     ms_per_step = dt * 1e3 / args.steps
     value = ne * cols / (dt / args.steps) / 1e9
@@ -929,10 +943,12 @@ def main():
                    "path": "plain (one instance per call)"},
         "phases_ms": dict(zip(["transpose", "intt_strided", "lde_block", "ntt_strided", "leaves", "tree_levels"],
                               [round(x, 4) for x in phase_ms])),
-        "roofline": roofline, "roofline_lde": roofline_lde, "with_h2d": with_h2d,
+        "roofline": roofline, "roofline_lde": roofline_lde,
         "root": ["%016x" % v for v in root],
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
+    if with_h2d is not None:
+        out["with_h2d"] = with_h2d
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)
