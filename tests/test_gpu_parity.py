@@ -474,6 +474,15 @@ def test_plain_large_commit_sampled(pc, oracle):
     lde2, nodes2 = pc.commit(d_trace, arity, nb, nbe, c, lde=lde, nodes=nodes)
     root2 = pc.MerkleTreeGL(arity, 0, True, ne, c, nodes=nodes2).get_root()
     assert (root1 == root2).all()
+    # the streamed commit from pinned host memory (eight groups of 16 columns)
+    # gives the same LDE and tree, word for word
+    print("large: streamed commit from host memory", flush=True)
+    h = torch.empty(n * c, dtype=torch.int64, pin_memory=True)
+    h.copy_(d_trace.cpu())
+    lde3, nodes3 = pc.commit_host(h, arity, nb, nbe, c)
+    torch.cuda.synchronize()
+    assert torch.equal(nodes3, nodes2)
+    assert torch.equal(lde3[:ne * c], lde2[:ne * c])
 
 
 # -------------------------------------------------------------- grinding --
