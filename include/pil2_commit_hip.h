@@ -131,9 +131,10 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
  * its columns of d_lde and absorbs it into the row sponges, whose capacity words
  * carry over in the leaf digests of d_nodes. d_lde and d_nodes end up equal to
  * pil2hip_commit's. d_stage: pil2hip_commit_host_stage_words() words. h_trace
- * should be pinned (hipHostMalloc) for the copies to run concurrently. Returns
- * once every copy has completed (h_trace and d_stage are then free); the LDE
- * and tree complete on `stream`. -33: groupCols not a multiple of the rate. */
+ * must be page-locked (hipHostMalloc): from pageable memory the runtime splits
+ * the 2D copies into small pieces. Returns once every copy has completed (h_trace
+ * and d_stage are then free); the LDE and tree complete on `stream`. -33:
+ * groupCols not a multiple of the rate; -34: h_trace not page-locked. */
 int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *h_trace, uint64_t *d_stage,
                             uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t groupCols,
                             void *stream);

@@ -346,6 +346,15 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
     if (!d_nodes || (nCols && (!d_lde || !h_trace || !d_stage))) return fail(-13, "null buffer");
     uint64_t rate = 4 * arity - 4, w = stream_group_cols(nBits, nCols, arity, groupCols);
     if (w < nCols && w % rate) return fail(-33, "groupCols must be a multiple of the sponge rate 4*arity-4");
+    if (nCols) {
+        // From pageable memory the runtime stages a 2D copy in small pieces and a
+        // 2^22-row group takes minutes, so only page-locked host memory is taken.
+        hipPointerAttribute_t pa;
+        hipError_t pe = hipPointerGetAttributes(&pa, h_trace);
+        (void)hipGetLastError();  // an unregistered pointer leaves an error that the launches below would report
+        if (pe != hipSuccess || pa.type != hipMemoryTypeHost)
+            return fail(-34, "h_trace must be page-locked host memory (hipHostMalloc)");
+    }
     hipStream_t st = (hipStream_t)stream;
     uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
     uint32_t width = (uint32_t)(4 * arity);

@@ -430,17 +430,13 @@ MUL_PEAK_CHAINS = 18
 
 def commit_host(h_trace, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, stage=None, group_cols=0,
                 stream=None):
-    """pil2hip_commit for a host trace (a pinned CPU tensor, or a numpy uint64
-    array), with the column groups' transfer overlapped. Returns (lde, nodes)."""
+    """pil2hip_commit for a host trace in page-locked memory (a contiguous pinned
+    CPU tensor), with the column groups' transfer overlapped. Returns (lde, nodes)."""
     torch = _torch()
     ne = 1 << n_bits_ext
-    if isinstance(h_trace, np.ndarray):
-        h = np.ascontiguousarray(h_trace, dtype=np.uint64).reshape(-1)
-        ptr, numel = h.ctypes.data, h.size
-    else:
-        if h_trace.is_cuda or not h_trace.is_contiguous():
-            raise ValueError("commit_host: h_trace must be a contiguous host tensor")
-        ptr, numel = h_trace.data_ptr(), h_trace.numel()
+    if not torch.is_tensor(h_trace) or h_trace.is_cuda or not h_trace.is_contiguous():
+        raise ValueError("commit_host: h_trace must be a contiguous pinned host tensor")
+    ptr, numel = h_trace.data_ptr(), h_trace.numel()
     if numel < (1 << n_bits) * n_cols:
         raise ValueError("commit_host: trace smaller than 2^n_bits * n_cols")
     dev = torch.device("cuda", torch.cuda.current_device())

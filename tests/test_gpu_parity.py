@@ -228,7 +228,7 @@ def test_commit_is_deterministic(pc):
 def test_commit_host_matches_device_commit(pc, nb, nbe, c, arity, gc):
     """The streamed host commit (column groups copied on their own stream and
     absorbed with the capacity carried between groups) gives pil2hip_commit's LDE
-    and tree, from pinned and from pageable host memory, on repeated calls."""
+    and tree, on repeated calls."""
     import torch
     print("commit_host case", nb, nbe, c, arity, gc, flush=True)
     n, ne = 1 << nb, 1 << nbe
@@ -239,8 +239,8 @@ def test_commit_host_matches_device_commit(pc, nb, nbe, c, arity, gc):
     want_lde, want_nodes = pc.to_host(lde_ref)[:ne * c], pc.to_host(nodes_ref)
     pinned = torch.empty(max(n * c, 1), dtype=torch.int64, pin_memory=True)
     pinned.copy_(d.cpu())
-    for src in (pinned, pc.to_host(d)[:n * c], pinned):
-        lde, nodes = pc.commit_host(src, arity, nb, nbe, c, group_cols=gc)
+    for _ in range(2):
+        lde, nodes = pc.commit_host(pinned, arity, nb, nbe, c, group_cols=gc)
         torch.cuda.synchronize()
         assert np.array_equal(pc.to_host(nodes), want_nodes)
         assert np.array_equal(pc.to_host(lde)[:ne * c], want_lde)
@@ -251,6 +251,24 @@ def test_commit_host_rejects_group_width_off_the_rate(pc):
     h = torch.zeros(1 << 10, dtype=torch.int64, pin_memory=True)
     with pytest.raises(pc.HipError, match="multiple of the sponge rate"):
         pc.commit_host(h, 3, 6, 7, 16, group_cols=5)
+
+
+def test_commit_host_rejects_pageable_memory(pc):
+    """A pageable host trace is refused before any copy (-34), and the next
+    call on pinned memory still works (no stale error from the pointer query)."""
+    import torch
+    n, c = 1 << 8, 16
+    pageable = torch.zeros(n * c, dtype=torch.int64)
+    with pytest.raises(pc.HipError, match="page-locked"):
+        pc.commit_host(pageable, 3, 8, 9, c)
+    pinned = torch.zeros(n * c, dtype=torch.int64, pin_memory=True)
+    lde, nodes = pc.commit_host(pinned, 3, 8, 9, c)
+    torch.cuda.synchronize()
+    d = pc.empty(n * c)
+    d.zero_()
+    lde_ref, nodes_ref = pc.commit(d, 3, 8, 9, c)
+    torch.cuda.synchronize()
+    assert torch.equal(nodes, nodes_ref)
 
 
 def test_fill_uniform_matches_datagen(pc):
