@@ -850,8 +850,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the commit path shards per job / per column group, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; runs on several GPUs are not part of this benchmark "
+              "(batched and served jobs shard per job, one large commit per column group)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes
