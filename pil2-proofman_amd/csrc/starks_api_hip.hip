@@ -361,6 +361,12 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
     int e = 0;
     if (nCols == 0) {  // empty rows: zero digests
         e = p2_launch_leaves(width, d_lde, 1, NExt, 0, NExt, d_nodes, st);
+    } else if (w >= nCols) {  // one group: a plain copy on the caller's stream, nothing to overlap
+        hipError_t he = hipMemcpyAsync(d_stage, h_trace, N * nCols * 8, hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = hipStreamSynchronize(st);  // h_trace is free on return
+        if (he != hipSuccess) return fail(-100 - (int64_t)he, "streamed commit: copy failed");
+        if ((rc = lde_impl(d_lde, d_stage, d_stage + 2 * N * w, nBits, nBitsExt, nCols, st, nullptr))) return rc;
+        e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
     } else {
         hipStream_t cs = nullptr;
         hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
