@@ -51,7 +51,7 @@ __host__ __device__ __forceinline__ lz lz_shl(lz a)
 
 #if defined(__HIP_DEVICE_COMPILE__)
 
-// r[k] = a[k] + b[k] for k < N (N = 1..4), chains interleaved.
+// r[k] = a[k] + b[k] for k < N (the permutation uses N = 1).
 template <int N>
 __device__ __forceinline__ void lz_add_n(lz *r, const lz *a, const lz *b);
 
@@ -66,64 +66,6 @@ __device__ __forceinline__ void lz_add_n<1>(lz *r, const lz *a, const lz *b)
         : "=&v"(r[0].l0), "=&v"(r[0].l1), "=&v"(r[0].c)
         : "v"(a[0].l0), "v"(a[0].l1), "v"(a[0].c), "v"(b[0].l0), "v"(b[0].l1), "v"(b[0].c)
         : "vcc");
-}
-
-template <>
-__device__ __forceinline__ void lz_add_n<2>(lz *r, const lz *a, const lz *b)
-{
-    uint64_t k;
-    asm("v_add_co_u32_e32 %0, vcc, %7, %10\n\t"
-        "v_add_co_u32_e64 %3, %[k], %13, %16\n\t"
-        "s_nop 0\n\t"
-        "v_addc_co_u32_e32 %1, vcc, %8, %11, vcc\n\t"
-        "v_addc_co_u32_e64 %4, %[k], %14, %17, %[k]\n\t"
-        "s_nop 0\n\t"
-        "v_addc_co_u32_e32 %2, vcc, %9, %12, vcc\n\t"
-        "v_addc_co_u32_e64 %5, %[k], %15, %18, %[k]"
-        : "=&v"(r[0].l0), "=&v"(r[0].l1), "=&v"(r[0].c), "=&v"(r[1].l0), "=&v"(r[1].l1), "=&v"(r[1].c),
-          [k] "=&s"(k)
-        : "v"(a[0].l0), "v"(a[0].l1), "v"(a[0].c), "v"(b[0].l0), "v"(b[0].l1), "v"(b[0].c), "v"(a[1].l0),
-          "v"(a[1].l1), "v"(a[1].c), "v"(b[1].l0), "v"(b[1].l1), "v"(b[1].c)
-        : "vcc");
-}
-
-template <>
-__device__ __forceinline__ void lz_add_n<3>(lz *r, const lz *a, const lz *b)
-{
-    uint64_t k1, k2;
-    asm("v_add_co_u32_e32 %0, vcc, %11, %14\n\t"
-        "v_add_co_u32_e64 %3, %[k1], %17, %20\n\t"
-        "v_add_co_u32_e64 %6, %[k2], %23, %26\n\t"
-        "v_addc_co_u32_e32 %1, vcc, %12, %15, vcc\n\t"
-        "v_addc_co_u32_e64 %4, %[k1], %18, %21, %[k1]\n\t"
-        "v_addc_co_u32_e64 %7, %[k2], %24, %27, %[k2]\n\t"
-        "v_addc_co_u32_e32 %2, vcc, %13, %16, vcc\n\t"
-        "v_addc_co_u32_e64 %5, %[k1], %19, %22, %[k1]\n\t"
-        "v_addc_co_u32_e64 %8, %[k2], %25, %28, %[k2]"
-        : "=&v"(r[0].l0), "=&v"(r[0].l1), "=&v"(r[0].c), "=&v"(r[1].l0), "=&v"(r[1].l1), "=&v"(r[1].c),
-          "=&v"(r[2].l0), "=&v"(r[2].l1), "=&v"(r[2].c), [k1] "=&s"(k1), [k2] "=&s"(k2)
-        : "v"(a[0].l0), "v"(a[0].l1), "v"(a[0].c), "v"(b[0].l0), "v"(b[0].l1), "v"(b[0].c), "v"(a[1].l0),
-          "v"(a[1].l1), "v"(a[1].c), "v"(b[1].l0), "v"(b[1].l1), "v"(b[1].c), "v"(a[2].l0), "v"(a[2].l1),
-          "v"(a[2].c), "v"(b[2].l0), "v"(b[2].l1), "v"(b[2].c)
-        : "vcc");
-}
-
-template <>
-__device__ __forceinline__ void lz_add_n<4>(lz *r, const lz *a, const lz *b)
-{
-    lz_add_n<2>(r, a, b);
-    lz_add_n<2>(r + 2, a + 2, b + 2);
-}
-
-// r[k] = a[k] + b[k] for k < n, in interleaved groups of three.
-template <int N>
-__device__ __forceinline__ void lz_add_all(lz *r, const lz *a, const lz *b)
-{
-    int k = 0;
-#pragma unroll
-    for (; k + 3 <= N; k += 3) lz_add_n<3>(r + k, a + k, b + k);
-    if constexpr (N % 3 == 1) lz_add_n<1>(r + N - 1, a + N - 1, b + N - 1);
-    if constexpr (N % 3 == 2) lz_add_n<2>(r + N - 2, a + N - 2, b + N - 2);
 }
 
 // lz -> congruent 64-bit word (c < 2^31)
@@ -143,62 +85,6 @@ __device__ __forceinline__ uint64_t lz_fold(lz a)
         : "vcc");
     return gl_pack(r0, r1);
 }
-
-// r[k] = a[k] + rc[k] for three lanes of round constants (wave-uniform): the
-// low halves of the constants are read from SGPRs directly (the carry-in steps
-// need VGPR operands, so only the high halves are copied).
-__device__ __forceinline__ void lz_add_rc3(lz *r, const lz *a, const uint64_t *rc)
-{
-    uint64_t k1, k2;
-    asm("v_add_co_u32_e32 %0, vcc, %11, %14\n\t"
-        "v_add_co_u32_e64 %3, %[k1], %12, %16\n\t"
-        "v_add_co_u32_e64 %6, %[k2], %13, %18\n\t"
-        "v_addc_co_u32_e32 %1, vcc, %15, %20, vcc\n\t"
-        "v_addc_co_u32_e64 %4, %[k1], %17, %21, %[k1]\n\t"
-        "v_addc_co_u32_e64 %7, %[k2], %19, %22, %[k2]\n\t"
-        "v_addc_co_u32_e32 %2, vcc, 0, %23, vcc\n\t"
-        "v_addc_co_u32_e64 %5, %[k1], 0, %24, %[k1]\n\t"
-        "v_addc_co_u32_e64 %8, %[k2], 0, %25, %[k2]"
-        : "=&v"(r[0].l0), "=&v"(r[0].l1), "=&v"(r[0].c), "=&v"(r[1].l0), "=&v"(r[1].l1), "=&v"(r[1].c),
-          "=&v"(r[2].l0), "=&v"(r[2].l1), "=&v"(r[2].c), [k1] "=&s"(k1), [k2] "=&s"(k2)
-        : "s"((uint32_t)rc[0]), "s"((uint32_t)rc[1]), "s"((uint32_t)rc[2]), "v"(a[0].l0), "v"(a[0].l1),
-          "v"(a[1].l0), "v"(a[1].l1), "v"(a[2].l0), "v"(a[2].l1), "v"((uint32_t)(rc[0] >> 32)),
-          "v"((uint32_t)(rc[1] >> 32)), "v"((uint32_t)(rc[2] >> 32)), "v"(a[0].c), "v"(a[1].c), "v"(a[2].c)
-        : "vcc");
-}
-
-// Three folds (lz_fold) with the carry fix-ups interleaved.
-__device__ __forceinline__ void lz_fold3(const lz *a, uint64_t *out)
-{
-    uint64_t r[3], cc[3];
-    asm("v_mad_u64_u32 %0, %3, %6, -1, %9\n\t"
-        "v_mad_u64_u32 %1, %4, %7, -1, %10\n\t"
-        "v_mad_u64_u32 %2, %5, %8, -1, %11"
-        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&s"(cc[0]), "=&s"(cc[1]), "=&s"(cc[2])
-        : "v"(a[0].c), "v"(a[1].c), "v"(a[2].c), "v"(gl_pack(a[0].l0, a[0].l1)), "v"(gl_pack(a[1].l0, a[1].l1)),
-          "v"(gl_pack(a[2].l0, a[2].l1)));
-    uint32_t r0[3], r1[3], t[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        r0[k] = (uint32_t)r[k];
-        r1[k] = (uint32_t)(r[k] >> 32);
-    }
-    // the mads above are at least two instructions before each mask read below
-    asm("v_cndmask_b32_e64 %6, 0, -1, %9\n\t"
-        "v_cndmask_b32_e64 %7, 0, -1, %10\n\t"
-        "v_cndmask_b32_e64 %8, 0, -1, %11\n\t"
-        "v_add_co_u32_e64 %0, %9, %0, %6\n\t"
-        "v_add_co_u32_e64 %1, %10, %1, %7\n\t"
-        "v_add_co_u32_e64 %2, %11, %2, %8\n\t"
-        "v_addc_co_u32_e64 %3, %9, 0, %3, %9\n\t"
-        "v_addc_co_u32_e64 %4, %10, 0, %4, %10\n\t"
-        "v_addc_co_u32_e64 %5, %11, 0, %5, %11"
-        : "+v"(r0[0]), "+v"(r0[1]), "+v"(r0[2]), "+v"(r1[0]), "+v"(r1[1]), "+v"(r1[2]), "=&v"(t[0]), "=&v"(t[1]),
-          "=&v"(t[2]), "+s"(cc[0]), "+s"(cc[1]), "+s"(cc[2]));
-#pragma unroll
-    for (int k = 0; k < 3; k++) out[k] = gl_pack(r0[k], r1[k]);
-}
-
 
 // Three halves-form values (lo, hi < 2^41) to congruent 64-bit words. With
 // hi = h0 + 2^32 h1 and 2^64 = 2^32 - 1 (mod p): r = lo + h1 (2^32 - 1) is
@@ -289,11 +175,8 @@ __device__ __forceinline__ void p2_mul_add3(const uint64_t *a, const uint64_t *b
 #else  // host pass: declarations only (the kernels are compiled for the device)
 
 template <int N> __device__ void lz_add_n(lz *r, const lz *a, const lz *b);
-template <int N> __device__ void lz_add_all(lz *r, const lz *a, const lz *b);
-__device__ void lz_add_rc3(lz *r, const lz *a, const uint64_t *rc);
 __device__ void p2_mul3(const uint64_t *a, const uint64_t *b, uint64_t *out);
 __device__ void p2_mul_add3(const uint64_t *a, const uint64_t *b, uint32_t s_lo, uint32_t s_hi, uint64_t *out);
-__device__ void lz_fold3(const lz *a, uint64_t *out);
 __device__ void hz_fold3(const hz *a, uint64_t *out);
 __device__ uint64_t hz_fold(hz a);
 __device__ uint64_t lz_fold(lz a);

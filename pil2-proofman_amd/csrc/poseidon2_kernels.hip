@@ -167,30 +167,6 @@ __device__ __forceinline__ void full_round(hz *t, int rc_base)
     ext_layer_hz<W>(s, t);
 }
 
-// Sum of N lazy values (pairwise tree, additions interleaved per level).
-template <int N>
-__device__ __forceinline__ lz sum_tree(const lz *v)
-{
-    if constexpr (N == 1) {
-        return v[0];
-    } else {
-        constexpr int H = N / 2;
-        lz r[H + (N & 1)];
-        lz_add_all<H>(r, v, v + H);
-        if constexpr (N & 1) r[H] = v[N - 1];
-        return sum_tree<H + (N & 1)>(r);
-    }
-}
-
-template <int W>
-__device__ __forceinline__ lz sum_lz(const uint64_t *s)
-{
-    lz v[W];
-#pragma unroll
-    for (int i = 0; i < W; i++) v[i] = lz_of(s[i]);
-    return sum_tree<W>(v);
-}
-
 // Sum of W words in halves form: the low and the high 32-bit halves are summed
 // separately, each as one 32-bit addition with carry and then a chain of
 // v_mad_u64_u32 (half * 1 + accumulator), which takes the 32-bit half as it is
