@@ -1,21 +1,22 @@
 // Non-canonical Goldilocks arithmetic for the Poseidon2 permutation (gfx950).
 //
 // Inside the permutation every value only has to be congruent mod p; it is
-// canonicalised once, when a digest leaves the kernel. Two forms are used:
+// canonicalised once, when a digest leaves the kernel. Three forms are used:
 //   u64 word  any 64-bit value v (meaning v mod p);
-//   lz        three 32-bit words (l0, l1, c) meaning l0 + 2^32 l1 + 2^64 c,
-//             with a small carry count c: the result of additions whose
-//             2^64-overflows are counted instead of reduced.
-// Since 2^64 = 2^32 - 1 (mod p), an lz value folds back to a word with one
-// v_mad_u64_u32: (l1:l0) + c*(2^32 - 1), plus one conditional +(2^32 - 1) when
-// that sum carries out (the carried remainder is < c*2^32, so it cannot carry
-// again).
+//   hz        halves form: two 64-bit accumulators (lo, hi) meaning lo + 2^32 hi,
+//             each below 2^41. The linear layers sum the 32-bit halves of words
+//             with small coefficients in it, with no carry chains at all;
+//   lz        three 32-bit words (l0, l1, c) meaning l0 + 2^32 l1 + 2^64 c with a
+//             small carry count c (round-constant additions and the cooperative
+//             kernels' M4).
+// Since 2^64 = 2^32 - 1 (mod p), both fold back to a word with one
+// v_mad_u64_u32 and a conditional +(2^32 - 1) on a carry-out.
 //
-// Additions run as 32-bit carry chains. gfx950 needs two wait states between
-// a VALU write of a carry mask and the VALU instruction reading it; the
-// batched adders below interleave three or four independent chains (carries in
-// vcc and SGPR pairs) so that every read is at least two instructions after
-// its write, and no s_nop is needed.
+// gfx950 needs two wait states between a VALU write of a carry mask and the
+// VALU instruction reading it; the three-lane helpers below interleave three
+// independent chains (carries in SGPR pairs) so that every read is at least two
+// instructions after its write, and the single-lane ones fill the gaps with
+// s_nop.
 //
 // Products are chains of v_mad_u64_u32 whose addends carry the partial sums
 // (no carry chains at all), reduced with one more v_mad_u64_u32 and a short
