@@ -7,8 +7,7 @@
 //             each below 2^41. The linear layers sum the 32-bit halves of words
 //             with small coefficients in it, with no carry chains at all;
 //   lz        three 32-bit words (l0, l1, c) meaning l0 + 2^32 l1 + 2^64 c with a
-//             small carry count c (round-constant additions and the cooperative
-//             kernels' M4).
+//             small carry count c (the partial rounds' round-constant addition).
 // Since 2^64 = 2^32 - 1 (mod p), both fold back to a word with one
 // v_mad_u64_u32 and a conditional +(2^32 - 1) on a carry-out.
 //
@@ -38,17 +37,6 @@ struct hz {
 };
 
 __host__ __device__ __forceinline__ lz lz_of(uint64_t v) { return {(uint32_t)v, (uint32_t)(v >> 32), 0u}; }
-
-// a * 2^s for s = 1, 2 (no carries: the count absorbs the shifted-out bits)
-template <int S>
-__host__ __device__ __forceinline__ lz lz_shl(lz a)
-{
-    lz r;
-    r.c = (a.c << S) | (a.l1 >> (32 - S));
-    r.l1 = (a.l1 << S) | (a.l0 >> (32 - S));
-    r.l0 = a.l0 << S;
-    return r;
-}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 

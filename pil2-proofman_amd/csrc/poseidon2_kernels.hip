@@ -311,16 +311,6 @@ __device__ __forceinline__ void dpp_ready(uint32_t &a, uint32_t &b, uint32_t &c)
     asm volatile("s_nop 1" : "+v"(a), "+v"(b), "+v"(c));
 }
 
-template <int CTRL>
-__device__ __forceinline__ lz dpp_lz(lz v)
-{
-    lz r;
-    r.l0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.l0, CTRL, 0xF, 0xF, false);
-    r.l1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.l1, CTRL, 0xF, 0xF, false);
-    r.c = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.c, CTRL, 0xF, 0xF, false);
-    return r;
-}
-
 // the same barrier for two 64-bit values
 __device__ __forceinline__ void dpp_ready64(uint64_t &a, uint64_t &b)
 {
@@ -345,26 +335,42 @@ __device__ __forceinline__ lz lz_add1(lz a, lz b)
     return r;
 }
 
-__device__ __forceinline__ lz lz_sel(bool c, lz a, lz b) { return c ? a : b; }
-
-// M_E on the row: lane i gets (M4 x_quad)[i mod 4] + its column sum.
-template <int W>
-__device__ __forceinline__ lz ext_coop(uint64_t x, uint32_t i, bool live)
+// M_E on the row in halves form: the quad's four words are broadcast as 32-bit
+// halves, M4 runs on each half in 64-bit sums, and the column sum over the
+// blocks is two DPP row rotations of the 64-bit accumulators. Lane i gets
+// (M4 x_quad)[i mod 4] + its column sum; lanes >= W get zero.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 {
-    lz v = lz_of(x);
-    dpp_ready(v.l0, v.l1, v.c);
-    lz x0 = dpp_lz<dpp_quad_bcast(0)>(v), x1 = dpp_lz<dpp_quad_bcast(1)>(v);
-    lz x2 = dpp_lz<dpp_quad_bcast(2)>(v), x3 = dpp_lz<dpp_quad_bcast(3)>(v);
-    lz t0 = lz_add1(x0, x1), t1 = lz_add1(x2, x3);
-    lz t2 = lz_add1(lz_shl<1>(x1), t1), t3 = lz_add1(lz_shl<1>(x3), t0);
-    lz t4 = lz_add1(lz_shl<2>(t1), t3), t5 = lz_add1(lz_shl<2>(t0), t2);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+
+template <int W>
+__device__ __forceinline__ hz ext_coop_hz(uint64_t x, uint32_t i, bool live)
+{
+    uint64_t xv = x, dummy = 0;
+    dpp_ready64(xv, dummy);
+    uint32_t xl = (uint32_t)xv, xh = (uint32_t)(xv >> 32);
     uint32_t q = i & 3;
-    lz y = q == 0 ? lz_add1(t3, t5) : q == 1 ? t5 : q == 2 ? lz_add1(t2, t4) : t4;
-    dpp_ready(y.l0, y.l1, y.c);
-    lz c = lz_add1(y, dpp_lz<dpp_row_ror(4)>(y));
-    dpp_ready(c.l0, c.l1, c.c);
-    c = lz_add1(c, dpp_lz<dpp_row_ror(8)>(c));
-    return lz_sel(live, lz_add1(y, c), lz_of(0));
+    uint64_t y[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint32_t v = h ? xh : xl;
+        uint64_t a0 = dpp_u32<dpp_quad_bcast(0)>(v), a1 = dpp_u32<dpp_quad_bcast(1)>(v);
+        uint64_t a2 = dpp_u32<dpp_quad_bcast(2)>(v), a3 = dpp_u32<dpp_quad_bcast(3)>(v);
+        uint64_t t0 = a0 + a1, t1 = a2 + a3;
+        uint64_t t2 = (a1 << 1) + t1, t3 = (a3 << 1) + t0;
+        uint64_t t4 = (t1 << 2) + t3, t5 = (t0 << 2) + t2;
+        y[h] = q == 0 ? t3 + t5 : q == 1 ? t5 : q == 2 ? t2 + t4 : t4;
+    }
+    hz c = {y[0], y[1]};
+    dpp_ready64(c.lo, c.hi);
+    c.lo += dpp_u64<dpp_row_ror(4)>(c.lo);
+    c.hi += dpp_u64<dpp_row_ror(4)>(c.hi);
+    dpp_ready64(c.lo, c.hi);
+    c.lo += dpp_u64<dpp_row_ror(8)>(c.lo);
+    c.hi += dpp_u64<dpp_row_ror(8)>(c.hi);
+    return live ? hz{y[0] + c.lo, y[1] + c.hi} : hz{0, 0};
 }
 
 template <int W>
@@ -372,13 +378,15 @@ __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t s, uint32_t i)
 {
     const bool live = i < (uint32_t)W;
     const uint32_t li = live ? i : 0;
-    lz t = ext_coop<W>(s, i, live);
+    hz t = ext_coop_hz<W>(s, i, live);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
-        lz rc = lz_of(live ? P2Tab<W>::rc(r * W + li) : 0);
-        t = ext_coop<W>(pow7(lz_fold(lz_add1(t, rc))), i, live);
+        uint64_t rc = live ? P2Tab<W>::rc(r * W + li) : 0;
+        t.lo += (uint32_t)rc;
+        t.hi += rc >> 32;
+        t = ext_coop_hz<W>(pow7(hz_fold(t)), i, live);
     }
-    s = lz_fold(t);
+    s = hz_fold(t);
     const uint64_t d = live ? P2Tab<W>::d(li) : 0;
 #pragma unroll 1
     for (int r = 0; r < 22; r++) {
@@ -400,13 +408,15 @@ __device__ __forceinline__ uint64_t p2_permute_coop(uint64_t s, uint32_t i)
         uint64_t sum = hz_fold(v);
         s = live ? p2_mul_add(s, d, (uint32_t)sum, (uint32_t)(sum >> 32)) : 0;
     }
-    t = lz_of(s);
+    t = {(uint32_t)s, s >> 32};
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
-        lz rc = lz_of(live ? P2Tab<W>::rc(4 * W + 22 + r * W + li) : 0);
-        t = ext_coop<W>(pow7(lz_fold(lz_add1(t, rc))), i, live);
+        uint64_t rc = live ? P2Tab<W>::rc(4 * W + 22 + r * W + li) : 0;
+        t.lo += (uint32_t)rc;
+        t.hi += rc >> 32;
+        t = ext_coop_hz<W>(pow7(hz_fold(t)), i, live);
     }
-    return lz_fold(t);
+    return hz_fold(t);
 }
 
 // One tree level, 16 lanes per parent (same layout and pad rule as k_p2_nodes).
