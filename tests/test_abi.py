@@ -69,6 +69,24 @@ def test_set_gpu_mode_reports_gpu_backend(lib):
     assert f(False) is False
 
 
+def test_commit_host_stage_words(lib):
+    """Staging size of the streamed host commit (pure host logic): three buffers of
+    2^nBits rows times the group width; one group under 256 MiB of trace, else
+    about eight groups of whole rate blocks and at least 16 columns."""
+    f = lib.pil2hip_commit_host_stage_words
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint64] * 4
+    assert f(22, 128, 3, 0) == 3 * (1 << 22) * 16      # 4 GiB: eight groups of 16
+    assert f(16, 32, 3, 0) == 3 * (1 << 16) * 32       # 16 MiB: one group
+    assert f(20, 64, 3, 0) == 3 * (1 << 20) * 16       # 512 MiB: four groups of 16
+    assert f(22, 300, 4, 0) == 3 * (1 << 22) * 48      # rate 12: ceil(25 / 8) = 4 blocks
+    assert f(22, 128, 2, 0) == 3 * (1 << 22) * 16      # rate 4: at least 16 columns
+    assert f(10, 40, 3, 8) == 3 * (1 << 10) * 8        # explicit width
+    assert f(10, 40, 3, 64) == 3 * (1 << 10) * 40      # clamped to the trace
+    assert f(10, 0, 3, 0) == 3 * (1 << 10)             # no columns: one word per row
+    assert f(10, 40, 5, 0) == 0                        # bad arity
+
+
 def test_batch_shape_plan_defaults_before_tuning(lib):
     """A shape the batch autotuner has not measured reports 0 and the default plan
     (pure host logic: no device call)."""
