@@ -504,8 +504,10 @@ int oracle_fri_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challeng
     uint64_t winvX = o_inv(O_W[lX]), nXinv = o_inv(nX);
     #pragma omp parallel for schedule(static)
     for (uint64_t g = 0; g < n2; g++) {
-        uint64_t buf[3][64];
-        uint64_t coef[64][3];
+        /* any fold ratio: per-group buffers of nX cubic values (buf[k] = bufs + k*nX) */
+        uint64_t *bufs = (uint64_t *)malloc(6 * nX * sizeof(uint64_t));
+        uint64_t *buf[3] = {bufs, bufs + nX, bufs + 2 * nX};
+        uint64_t (*coef)[3] = (uint64_t (*)[3])(bufs + 3 * nX);
         for (uint64_t i = 0; i < nX; i++)
             for (int k = 0; k < 3; k++) buf[k][i] = pol[(i * n2 + g) * 3 + k];
         for (int k = 0; k < 3; k++) {
@@ -524,6 +526,7 @@ int oracle_fri_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challeng
             for (int k = 0; k < 3; k++) acc[k] = o_add(t[k], coef[i][k]);
         }
         for (int k = 0; k < 3; k++) out[g * 3 + k] = acc[k];
+        free(bufs);
     }
     return 0;
 }
@@ -550,9 +553,9 @@ void oracle_fri_fold_one(uint64_t *out3, const uint64_t *vals, const uint64_t *c
     uint64_t shift = 7;
     for (uint64_t j = 0; j < nBitsExt - prevBits; j++) shift = o_mul(shift, shift);
     uint64_t sinv = o_inv(o_mul(shift, o_pow(O_W[prevBits], g)));
-    uint64_t coef[64][3];
+    uint64_t *mem = (uint64_t *)malloc(5 * nX * sizeof(uint64_t));
+    uint64_t (*coef)[3] = (uint64_t (*)[3])mem, *col = mem + 3 * nX, *res = mem + 4 * nX;
     for (int k = 0; k < 3; k++) {
-        uint64_t col[64], res[64];
         for (uint64_t i = 0; i < nX; i++) col[i] = vals[i * 3 + k];
         oracle_dft_naive(res, col, nX, 1);
         for (uint64_t i = 0; i < nX; i++) coef[i][k] = res[i];
@@ -566,6 +569,7 @@ void oracle_fri_fold_one(uint64_t *out3, const uint64_t *vals, const uint64_t *c
         for (int k = 0; k < 3; k++) acc[k] = o_add(t[k], coef[i][k]);
     }
     memcpy(out3, acc, 24);
+    free(mem);
 }
 
 /* --------------------------------------------------------------- helpers -- */

@@ -266,7 +266,7 @@ def test_merkle_openings_recompute_root(oracle, arity, llv):
 
 
 # -------------------------------------------------------------------- FRI --
-@pytest.mark.parametrize("prev,cur,nbe", [(10, 7, 12), (9, 7, 9), (8, 5, 10), (6, 5, 8)])
+@pytest.mark.parametrize("prev,cur,nbe", [(10, 7, 12), (9, 7, 9), (8, 5, 10), (6, 5, 8), (11, 4, 12), (12, 3, 12)])
 def test_fri_fold_matches_per_coset_bruteforce(oracle, prev, cur, nbe):
     pol = datagen.uniform(3 << prev, 1000 + prev)
     ch = datagen.uniform(3, 5)
@@ -275,6 +275,23 @@ def test_fri_fold_matches_per_coset_bruteforce(oracle, prev, cur, nbe):
     for g in [0, 1, 5, n2 - 1]:
         vals = np.stack([pol[(i * n2 + g) * 3:(i * n2 + g) * 3 + 3] for i in range(nx)])
         assert (out[g * 3:g * 3 + 3] == oracle.fri_fold_one(vals, ch, nbe, prev, cur, g)).all()
+
+
+@pytest.mark.parametrize("prev,cur,nbe", [(12, 5, 14), (11, 1, 11), (13, 3, 13)])
+def test_fri_fold_large_ratio_is_two_folds(oracle, prev, cur, nbe):
+    """A fold of ratio 2^k (k > 5) equals a fold of ratio 32 at the challenge c
+    followed by a fold of ratio 2^(k-5) at c^32: with f(x) = sum_i x^i F_i(x^nX)
+    and i = j + 32 m, sum_i c^i F_i = sum_m (c^32)^m sum_j c^j F_(j+32m). The
+    device folds ratios 2^6..2^10 this way; the brute-force test above pins the
+    oracle's direct fold at such ratios."""
+    rng = np.random.default_rng(prev * 31 + cur)
+    pol = np.array(rng.integers(0, P, 3 << prev, dtype=np.uint64))
+    c = np.array(rng.integers(0, P, 3, dtype=np.uint64))
+    c32 = c.copy()
+    for _ in range(5):
+        c32 = oracle.c3_mul(c32, c32)
+    mid = oracle.fri_fold(pol, c, 1, nbe, prev, prev - 5)
+    assert np.array_equal(oracle.fri_fold(pol, c, 1, nbe, prev, cur), oracle.fri_fold(mid, c32, 1, nbe, prev - 5, cur))
 
 
 def test_fri_fold_is_polynomial_evaluation(oracle):
@@ -303,14 +320,27 @@ def test_fri_fold_is_polynomial_evaluation(oracle):
         assert int(out[3 * g]) == expect and int(out[3 * g + 1]) == 0
 
 
-def test_fri_transpose(oracle):
-    deg, tb = 64, 3
-    pol = np.arange(3 * deg, dtype=np.uint64)
-    aux = oracle.fri_transpose(pol, deg, tb)
-    w, h = 1 << tb, deg >> tb
-    for i in range(w):
-        for j in range(h):
-            assert (aux[(i * h + j) * 3:(i * h + j) * 3 + 3] == pol[(j * w + i) * 3:(j * w + i) * 3 + 3]).all()
+@pytest.mark.parametrize("prev,cur,nbe", [(8, 5, 10), (9, 5, 9), (7, 6, 12), (10, 5, 13), (6, 1, 8)])
+def test_fri_transpose_rows_verify_the_fold(oracle, prev, cur, nbe):
+    """getTransposed (fri.hpp:209-226) is what FRI::merkelize commits, and a
+    query opens row g of that tree. The verifier (verify_fold, fri.hpp:229-256;
+    fields/src/fri.rs:137-184) interpolates the opened row and evaluates it at
+    challenge * sinv_g, which must give the next layer's value at g. So row g of
+    the regrouped layer, folded by fri_fold_one, equals fold(pol)[g] from the
+    whole-layer fold that per-coset brute force pins above. A regrouping with a
+    transposed or shifted index puts other words into the row and fails."""
+    rng = np.random.default_rng(prev * 100 + cur)
+    pol = np.array(rng.integers(0, P, 3 << prev, dtype=np.uint64))
+    ch = np.array(rng.integers(0, P, 3, dtype=np.uint64))
+    folded = oracle.fri_fold(pol, ch, 1, nbe, prev, cur)
+    aux = oracle.fri_transpose(pol, 1 << prev, cur)
+    nx = 1 << (prev - cur)
+    assert aux.size == pol.size
+    for g in range(1 << cur):
+        row = aux[g * 3 * nx:(g + 1) * 3 * nx]
+        assert np.array_equal(oracle.fri_fold_one(row, ch, nbe, prev, cur, g), folded[3 * g:3 * g + 3]), g
+    # the regrouping is a permutation of whole cubic elements
+    assert sorted(map(tuple, aux.reshape(-1, 3).tolist())) == sorted(map(tuple, pol.reshape(-1, 3).tolist()))
 
 
 # --------------------------------------------------------------- datagen --
@@ -458,6 +488,34 @@ def run_transcript(t, ops):
         else:
             out.append([int(x) for x in t.get_permutations(*arg)])
     return out
+
+
+@pytest.mark.parametrize("width", [8, 12, 16])
+def test_transcript_reproduces_linear_hash_goldens(oracle, refvec, width):
+    """TranscriptGL (transcriptGL.cpp:4-118) starts from a zero state and absorbs
+    RATE = W-4 pending words plus the 4 state words per permutation, zero-padding
+    the last block: the same sponge as linear_hash_seq (poseidon2_goldilocks.cpp:
+    43-77). So put([1..20]) then getState(4) is the reference's linearHash golden
+    of [1..20] (test_helpers.hpp:54-59). When words are still pending, the first
+    getField squeezes out[0..3) of that same final permutation; and
+    getPermutations draws its indices from the low 63 bits of those squeezed
+    words, least significant bit first (checked in closed form on the golden)."""
+    golden = hexes(refvec["linear_hash_w%d_1to20" % width])
+    vals = np.arange(1, 21, dtype=np.uint64)
+    t = oracle.transcript(width // 4)
+    t.put(vals)
+    assert [int(x) for x in t.get_state(4)] == golden
+    rate = width - 4
+    if 20 % rate:  # pending words: the squeeze runs the final block
+        t = oracle.transcript(width // 4)
+        t.put(vals)
+        assert [int(x) for x in t.get_field()] == golden[:3]
+        t = oracle.transcript(width // 4)
+        t.put(vals)
+        n, nb = 3, 40  # 120 bits: field 0 gives 63 bits, field 1 the rest
+        bits = (golden[0] & ((1 << 63) - 1)) | ((golden[1] & ((1 << 63) - 1)) << 63)
+        expect = [(bits >> (nb * i)) & ((1 << nb) - 1) for i in range(n)]
+        assert [int(x) for x in t.get_permutations(n, nb)] == expect
 
 
 @pytest.mark.parametrize("arity", [2, 3, 4])
