@@ -50,6 +50,25 @@ void set_hash_family(uint8_t fam);
 void write_custom_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
                          void *d_buffers, void *buffer, char *bufferFile);
 
+/* commit_witness (starks_api.hpp:98; CPU impl commit_witness_cpu,
+ * starks_api.cpp:683-723) for an unpacked trace whose witness columns are already
+ * computed (no packed cm1, no witness hints: the SetupCtx-dependent steps before
+ * the LDE stay with the caller). The host trace (row-major 2^nBits x nCols, the
+ * "cm1" stage-1 section) is extended and merkelized on the GPU, and the outputs
+ * land where commit_witness_cpu leaves them for the later stages and openings:
+ *   aux_trace[offset_dst ..]  the row-major LDE, 2^nBitsExt x nCols ("cm1" extended)
+ *   aux_trace[offset_mt ..]   the Merkle nodes, pil2hip_tree_num_elements words ("mt1")
+ *   root                      4 words (MerkleTreeGL::getRoot)
+ * The numbers are the ones commit_witness_cpu reads from its SetupCtx:
+ * starkStruct.nBits, nBitsExt and merkleTreeArity, mapSectionsN["cm1"],
+ * mapOffsets[("cm1", true)] and mapOffsets[("mt1", true)]. Nothing else in
+ * aux_trace is written. d_buffers: NULL or a gen_device_buffers handle (its
+ * stream is used). Synchronous. Returns 0, or a negative code (-35: the two
+ * sections overlap). */
+int64_t pil2hip_commit_witness(uint64_t *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                               const uint64_t *trace, uint64_t *aux_trace, uint64_t offset_dst, uint64_t offset_mt,
+                               void *d_buffers);
+
 /* starks_api.hpp:190 / :193. Returns an opaque handle that owns a HIP stream on
  * the current device (default priority). node_rank/node_size/numa_nodes are
  * accepted for signature compatibility; this build uses one GPU per process. */
@@ -132,9 +151,12 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
  * carry over in the leaf digests of d_nodes. d_lde and d_nodes end up equal to
  * pil2hip_commit's. d_stage: pil2hip_commit_host_stage_words() words. h_trace
  * must be page-locked (hipHostMalloc): from pageable memory the runtime splits
- * the 2D copies into small pieces. Returns once every copy has completed (h_trace
- * and d_stage are then free); the LDE and tree complete on `stream`. -33:
- * groupCols not a multiple of the rate; -34: h_trace not page-locked. */
+ * the 2D copies into small pieces. Returns once every host-to-device copy has
+ * completed, so h_trace is free on return. d_stage is NOT free on return: the
+ * LDE of the last column group still reads it on `stream`, so it may be reused
+ * or freed only by work ordered after the commit on `stream` (or after
+ * synchronising it). The LDE and tree complete on `stream`. -33: groupCols not a
+ * multiple of the rate; -34: h_trace not page-locked. */
 int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *h_trace, uint64_t *d_stage,
                             uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, uint64_t groupCols,
                             void *stream);
@@ -353,9 +375,19 @@ void pil2hip_serve_destroy(void *srv);
  * (4 words) into host memory. */
 int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, void *stream);
 
+/* MerkleTreeGL::getLevel (merkleTreeGL.cpp:84-101), the last_levels that
+ * FRI::merkelize stores (fri.hpp:108): the nodes of the first tree level with at
+ * most arity^llv entries, copied into h_level (host memory) and zero-padded to
+ * arity^llv digests, i.e. pil2hip_level_words(arity, llv) words. llv = 0 leaves
+ * h_level untouched, as the reference does. Synchronous on `stream`. */
+uint64_t pil2hip_level_words(uint64_t arity, uint64_t llv);
+int64_t pil2hip_get_level(uint64_t *h_level, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+                          void *stream);
+
 /* FRI::fold (fri.hpp:32-99) for step > 0: d_out[g] (g < 2^currentBits, cubic,
  * 3 words each) from d_pol (2^prevBits cubic values). challenge: 3 words in host
- * memory. d_out may equal d_pol (in place, as the reference). step == 0 is a no-op. */
+ * memory. d_out may equal d_pol (in place, as the reference). step == 0 is a no-op.
+ * Fold ratios 2^1 .. 2^10 (prevBits - currentBits <= 10; -17 above that). */
 int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t *challenge, uint64_t step,
                          uint64_t nBitsExt, uint64_t prevBits, uint64_t currentBits, void *stream);
 
@@ -370,7 +402,7 @@ int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t d
  * fri.hpp:102-109): getTransposed into d_sources[s] (2^steps[s] cubic values)
  * and merkelize into d_nodes[s] (pil2hip_tree_num_elements(2^steps[s+1], arity)
  * words), i.e. a tree of 2^steps[s+1] rows of 3*2^(steps[s]-steps[s+1]) words.
- * steps[0] must equal nBitsExt and each step drops 1 .. 5 bits
+ * steps[0] must equal nBitsExt and each step drops 1 .. 10 bits
  * (stark_struct.rs:162-166 gives the schedule). graph = true captures the whole
  * sequence once per argument set (all pointers, steps, arity) and replays the
  * cached graph on later calls, so new challenges written into d_challenges are

@@ -12,34 +12,15 @@
 #include "gl_field.h"
 #include "fri_kernels.h"
 
-// The challenge is (ch0, ch1, ch2), or the three words at chp when chp is not
-// null (device memory read at run time, so a captured graph folds with the
-// buffer's current contents).
+// Fold of one group from its NX = 2^LOGX cubic values a[k][i] (component k of
+// the i-th value pol[i*2^cur + g], canonical): inverse DFT of size NX, scale
+// coefficient i by nx_inv * sinv^i, Horner at x. Returns the folded value.
 template <int LOGX>
-__global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, const uint64_t *__restrict__ pol,
-                                                  uint32_t cur_bits, uint64_t shift_inv, uint64_t w_inv,
-                                                  uint64_t ch0, uint64_t ch1, uint64_t ch2,
-                                                  const uint64_t *__restrict__ chp, uint64_t wx_inv,
-                                                  uint64_t nx_inv)
+__device__ __forceinline__ gl3 fold_group(uint64_t (&a)[3][1 << LOGX], uint64_t sinv, gl3 x, uint64_t wx_inv,
+                                          uint64_t nx_inv)
 {
     constexpr int NX = 1 << LOGX;
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t n2 = 1ull << cur_bits;
-    if (g >= n2) return;
-    if (chp) {
-        ch0 = gl_canon(chp[0]);
-        ch1 = gl_canon(chp[1]);
-        ch2 = gl_canon(chp[2]);
-    }
-    uint64_t a[3][NX];
-#pragma unroll
-    for (int i = 0; i < NX; i++) {
-        // bit-reversed placement so the DIT below yields natural order
-        int p = (int)bitrev32((uint32_t)i, LOGX);
-#pragma unroll
-        for (int k = 0; k < 3; k++) a[k][p] = gl_canon(pol[((uint64_t)i * n2 + g) * 3 + k]);
-    }
-    // inverse DFT (radix-2 DIT, roots of w_NX^-1)
+    // inverse DFT (radix-2 DIT over the bit-reversed placement, roots of w_NX^-1)
 #pragma unroll
     for (int t = 0; t < LOGX; t++) {
         int h = 1 << t;
@@ -62,8 +43,7 @@ __global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, co
             w = gl_mul(w, wstep);
         }
     }
-    // sinv_g = shift_inv * w_inv^g ; coefficient i scaled by nx_inv * sinv_g^i
-    uint64_t sinv = gl_mul(shift_inv, gl_pow(w_inv, g));
+    // coefficient i scaled by nx_inv * sinv^i (polMulAxi), then evalPol at x
     uint64_t r = nx_inv;
 #pragma unroll
     for (int i = 0; i < NX; i++) {
@@ -71,15 +51,104 @@ __global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, co
         for (int k = 0; k < 3; k++) a[k][i] = gl_mul(a[k][i], r);
         r = gl_mul(r, sinv);
     }
-    gl3 x = {{ch0, ch1, ch2}};
     gl3 acc = {{a[0][NX - 1], a[1][NX - 1], a[2][NX - 1]}};
 #pragma unroll
     for (int i = NX - 2; i >= 0; i--) {
         gl3 c = {{a[0][i], a[1][i], a[2][i]}};
         acc = gl3_add(gl3_mul(acc, x), c);
     }
+    return acc;
+}
+
+// Loads group g's values pol[i*n2 + g] (i < 2^LOGX) in bit-reversed order, so the
+// DIT of fold_group yields natural order.
+template <int LOGX>
+__device__ __forceinline__ void load_group(uint64_t (&a)[3][1 << LOGX], const uint64_t *__restrict__ pol, uint64_t n2,
+                                           uint64_t g)
+{
 #pragma unroll
-    for (int k = 0; k < 3; k++) out[g * 3 + k] = acc.v[k];
+    for (int i = 0; i < (1 << LOGX); i++) {
+        int p = (int)bitrev32((uint32_t)i, LOGX);
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[k][p] = gl_canon(pol[((uint64_t)i * n2 + g) * 3 + k]);
+    }
+}
+
+__device__ __forceinline__ gl3 load_challenge(uint64_t ch0, uint64_t ch1, uint64_t ch2, const uint64_t *chp)
+{
+    if (chp) return {{gl_canon(chp[0]), gl_canon(chp[1]), gl_canon(chp[2])}};
+    return {{ch0, ch1, ch2}};
+}
+
+// The challenge is (ch0, ch1, ch2), or the three words at chp when chp is not
+// null (device memory read at run time, so a captured graph folds with the
+// buffer's current contents).
+template <int LOGX>
+__global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, const uint64_t *__restrict__ pol,
+                                                  uint32_t cur_bits, uint64_t shift_inv, uint64_t w_inv,
+                                                  uint64_t ch0, uint64_t ch1, uint64_t ch2,
+                                                  const uint64_t *__restrict__ chp, uint64_t wx_inv,
+                                                  uint64_t nx_inv)
+{
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n2 = 1ull << cur_bits;
+    if (g >= n2) return;
+    gl3 x = load_challenge(ch0, ch1, ch2, chp);
+    uint64_t a[3][1 << LOGX];
+    load_group<LOGX>(a, pol, n2, g);
+    // sinv_g = shift_inv * w_inv^g
+    gl3 r = fold_group<LOGX>(a, gl_mul(shift_inv, gl_pow(w_inv, g)), x, wx_inv, nx_inv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[g * 3 + k] = r.v[k];
+}
+
+// Fold ratios 2^6 .. 2^10 as two folds in one kernel. With x^nX = (x^32)^(nX/32),
+// the FRI fold of ratio nX at challenge c equals a fold of ratio 32 at c
+// (prev -> prev-5 bits, on the coset shift s and w_{2^prev}) followed by a fold of
+// ratio nB = nX/32 at c^32 (prev-5 -> cur bits, on the coset s^32 and
+// w_{2^(prev-5)}): the polynomial sum_i c^i f_i(y) regrouped as i = j + 32 l. A
+// block handles GB = 256/nB consecutive groups g: its 256 lanes first fold the
+// ratio-32 groups h = l*2^cur + g (l < nB) of the intermediate layer into LDS,
+// then GB lanes fold each g's nB intermediate values. Every read of pol comes
+// before the block's writes of out[g], and other blocks read only other g's
+// groups, so out may equal pol (in place, as the reference folds).
+template <int LOGB>
+__global__ __launch_bounds__(256) void k_fri_fold_big(uint64_t *__restrict__ out, const uint64_t *pol,
+                                                      uint32_t cur_bits, uint64_t shift_inv1, uint64_t w_inv1,
+                                                      uint64_t shift_inv2, uint64_t w_inv2, uint64_t ch0, uint64_t ch1,
+                                                      uint64_t ch2, const uint64_t *__restrict__ chp, uint64_t w32_inv,
+                                                      uint64_t n32_inv, uint64_t wb_inv, uint64_t nb_inv)
+{
+    constexpr int NB = 1 << LOGB, GB = 256 / NB;
+    __shared__ uint64_t mid[3][256];  // [k][l*GB + gl]
+    uint64_t n2 = 1ull << cur_bits, n1 = n2 << LOGB;  // intermediate layer: 2^(cur+LOGB) values
+    gl3 c = load_challenge(ch0, ch1, ch2, chp);
+    uint32_t t = threadIdx.x, l = t / GB, gl = t % GB;
+    uint64_t g = (uint64_t)blockIdx.x * GB + gl;
+    if (g < n2) {
+        uint64_t h = (uint64_t)l * n2 + g;
+        uint64_t a[3][32];
+        load_group<5>(a, pol, n1, h);
+        gl3 r = fold_group<5>(a, gl_mul(shift_inv1, gl_pow(w_inv1, h)), c, w32_inv, n32_inv);
+#pragma unroll
+        for (int k = 0; k < 3; k++) mid[k][t] = r.v[k];
+    }
+    __syncthreads();
+    if (t < (uint32_t)GB && g < n2) {
+        gl3 c32 = c;
+#pragma unroll
+        for (int q = 0; q < 5; q++) c32 = gl3_mul(c32, c32);
+        uint64_t a[3][NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            int p = (int)bitrev32((uint32_t)i, LOGB);
+#pragma unroll
+            for (int k = 0; k < 3; k++) a[k][p] = mid[k][i * GB + t];
+        }
+        gl3 r = fold_group<LOGB>(a, gl_mul(shift_inv2, gl_pow(w_inv2, g)), c32, wb_inv, nb_inv);
+#pragma unroll
+        for (int k = 0; k < 3; k++) out[g * 3 + k] = r.v[k];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fri_transpose(uint64_t *__restrict__ aux, const uint64_t *__restrict__ pol,
@@ -121,11 +190,36 @@ extern "C" int fri_launch_fold2(uint64_t *out, const uint64_t *pol, const uint64
         2530564950562219707ULL, 1905180297017055339ULL, 3524815499551269279ULL,
         7277203076849721926ULL};
     uint64_t w_inv = gl_inv(W[prev_bits]);
-    uint64_t wx_inv = gl_inv(W[logx]);
-    uint64_t nx_inv = gl_inv((1ull << logx) % GL_P);
     uint64_t c0 = 0, c1 = 0, c2 = 0;
     if (!d_challenge) c0 = gl_canon(challenge[0]), c1 = gl_canon(challenge[1]), c2 = gl_canon(challenge[2]);
     uint64_t n2 = 1ull << cur_bits;
+    if (logx > 5) {
+        // two-stage fold: ratio 32 (prev -> prev-5) at c, then ratio 2^(logx-5) at c^32
+        uint32_t logb = logx - 5;
+        uint64_t shift2 = shift;
+        for (int q = 0; q < 5; q++) shift2 = gl_mul(shift2, shift2);
+        uint64_t groups = 256u >> logb, blocks = (n2 + groups - 1) / groups;
+        dim3 gb((unsigned)blocks);
+        uint64_t w32_inv = gl_inv(W[5]), n32_inv = gl_inv(32), wb_inv = gl_inv(W[logb]);
+        uint64_t nb_inv = gl_inv(1ull << logb), w_inv2 = gl_inv(W[prev_bits - 5]), shift_inv2 = gl_inv(shift2);
+#define FOLD_BIG(L)                                                                                              \
+    case L:                                                                                                      \
+        hipLaunchKernelGGL(k_fri_fold_big<L>, gb, dim3(256), 0, st, out, pol, (uint32_t)cur_bits, shift_inv,     \
+                           w_inv, shift_inv2, w_inv2, c0, c1, c2, d_challenge, w32_inv, n32_inv, wb_inv, nb_inv); \
+        break;
+        switch (logb) {
+            FOLD_BIG(1)
+            FOLD_BIG(2)
+            FOLD_BIG(3)
+            FOLD_BIG(4)
+            FOLD_BIG(5)
+        default: return -2;
+        }
+#undef FOLD_BIG
+        return (int)hipGetLastError();
+    }
+    uint64_t wx_inv = gl_inv(W[logx]);
+    uint64_t nx_inv = gl_inv((1ull << logx) % GL_P);
     dim3 g((unsigned)((n2 + 255) / 256));
 #define FOLD_CASE(L)                                                                                        \
     case L:                                                                                                 \

@@ -371,7 +371,8 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
         hipStream_t cs = nullptr;
         hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
         auto done = [&](int64_t r) {
-            if (cs) (void)hipStreamSynchronize(cs);  // the host rows and staging are free after this
+            // the host rows are free after this; d_stage stays in use until `st` has run the last group's LDE
+            if (cs) (void)hipStreamSynchronize(cs);
             if (r) (void)hipStreamSynchronize(st);   // on an error nothing queued may still read d_stage
             for (int k = 0; k < 2; k++) {
                 if (copied[k]) (void)hipEventDestroy(copied[k]);
@@ -490,7 +491,31 @@ struct BatchGraph {
     std::vector<uint64_t> key;
     Pil2CommitPlan plan;
     hipGraphExec_t exec;
+    hipEvent_t done;  // recorded after every launch of exec; waited on before exec is destroyed
 };
+
+// A cached executable graph may still run on the caller's stream when it is
+// evicted or released: record an event after every launch and wait for it
+// before the exec is destroyed. A launch into a stream that is being captured
+// records no event (the caller's graph then holds its own copy of the work).
+static hipError_t launch_cached(hipGraphExec_t ex, hipEvent_t done, hipStream_t st)
+{
+    hipError_t e = hipGraphLaunch(ex, st);
+    if (e != hipSuccess) return e;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone)
+        e = hipEventRecord(done, st);
+    return e;
+}
+
+static void retire_exec(hipGraphExec_t ex, hipEvent_t done)
+{
+    if (done) {
+        (void)hipEventSynchronize(done);
+        (void)hipEventDestroy(done);
+    }
+    if (ex) (void)hipGraphExecDestroy(ex);
+}
 static std::vector<BatchGraph> g_batch_graphs;
 static std::vector<hipStream_t> g_batch_streams;
 static std::vector<hipEvent_t> g_batch_events;
@@ -719,6 +744,7 @@ static int64_t batch_try(const Pil2CommitJob *jobs, uint64_t n, const Pil2Commit
     if (te == hipSuccess) te = hipEventSynchronize(e1);
     if (te == hipSuccess) te = hipEventElapsedTime(ms, e0, e1);
     if (te != hipSuccess) {
+        (void)hipStreamSynchronize(st);  // a launch may have been queued before the failure
         (void)hipGraphExecDestroy(*ex);
         *ex = nullptr;
         return fail(-100 - (int64_t)te, std::string("autotune: ") + hipGetErrorString(te));
@@ -773,7 +799,7 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
     };
     for (auto &g : g_batch_graphs)
         if (g.key == key && matches(g.plan)) {
-            HIP_TRY(hipGraphLaunch(g.exec, st), "graph launch");
+            HIP_TRY(launch_cached(g.exec, g.done, st), "graph launch");
             if (chosen) *chosen = g.plan;
             return g.plan.n_streams;
         }
@@ -827,12 +853,18 @@ int64_t pil2hip_commit_batch_plan(const Pil2CommitJob *jobs, uint64_t n_jobs, co
         if (best) (void)hipGraphExecDestroy(best);
         return rc;
     }
-    if (g_batch_graphs.size() >= 8) {  // small cache: drop the oldest graph
-        (void)hipGraphExecDestroy(g_batch_graphs.front().exec);
+    hipEvent_t done = nullptr;
+    hipError_t de = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (de != hipSuccess) {
+        (void)hipGraphExecDestroy(best);
+        return fail(-100 - (int64_t)de, "graph event");
+    }
+    if (g_batch_graphs.size() >= 8) {  // small cache: drop the oldest graph once its last launch has ended
+        retire_exec(g_batch_graphs.front().exec, g_batch_graphs.front().done);
         g_batch_graphs.erase(g_batch_graphs.begin());
     }
-    g_batch_graphs.push_back({key, best_plan, best});
-    HIP_TRY(hipGraphLaunch(best, st), "graph launch");
+    g_batch_graphs.push_back({key, best_plan, best, done});
+    HIP_TRY(launch_cached(best, done, st), "graph launch");
     if (chosen) *chosen = best_plan;
     return best_plan.n_streams;
 }
@@ -850,7 +882,7 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
 void pil2hip_batch_release(void)
 {
     std::lock_guard<std::mutex> lk(g_batch_mu);
-    for (auto &g : g_batch_graphs) (void)hipGraphExecDestroy(g.exec);
+    for (auto &g : g_batch_graphs) retire_exec(g.exec, g.done);
     g_batch_graphs.clear();
 }
 
@@ -1078,6 +1110,41 @@ int64_t pil2hip_get_root(uint64_t *root, const uint64_t *d_nodes, uint64_t nRows
     return 0;
 }
 
+// MerkleTreeGL::getLevel (merkleTreeGL.cpp:84-101): the first level with at most
+// arity^llv nodes, zero-padded to arity^llv digests (what FRI::merkelize stores as
+// last_levels, fri.hpp:108). llv = 0 leaves h_level untouched, as the reference.
+uint64_t pil2hip_level_words(uint64_t arity, uint64_t llv)
+{
+    if (arity < 2 || llv == 0) return 0;
+    uint64_t lim = 1;
+    for (uint64_t i = 0; i < llv; i++) {
+        if (lim > (UINT64_MAX >> 2) / arity) return 0;
+        lim *= arity;
+    }
+    return lim * 4;
+}
+
+int64_t pil2hip_get_level(uint64_t *h_level, const uint64_t *d_nodes, uint64_t nRows, uint64_t arity, uint64_t llv,
+                          void *stream)
+{
+    int64_t rc = check_arity(arity);
+    if (rc) return rc;
+    if (llv == 0) return 0;
+    uint64_t words = pil2hip_level_words(arity, llv);
+    if (!words) return fail(-15, "arity^llv out of range");
+    if (!h_level || !d_nodes) return fail(-13, "null buffer");
+    uint64_t lim = words / 4, n = nRows, offset = 0;
+    while (n > lim) {
+        n = (n + arity - 1) / arity;
+        offset += n * arity * 4;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (n) HIP_TRY(hipMemcpyAsync(h_level, d_nodes + offset, n * 4 * 8, hipMemcpyDeviceToHost, st), "level copy");
+    HIP_TRY(hipStreamSynchronize(st), "level sync");
+    for (uint64_t i = n * 4; i < words; i++) h_level[i] = 0;
+    return 0;
+}
+
 int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t *challenge, uint64_t step,
                          uint64_t nBitsExt, uint64_t prevBits, uint64_t currentBits, void *stream)
 {
@@ -1085,7 +1152,7 @@ int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t 
     if (!challenge) return fail(-13, "null challenge");
     int e = fri_launch_fold(d_out, d_pol, challenge, nBitsExt, prevBits, currentBits, (hipStream_t)stream);
     if (e == -1) return fail(-16, "invalid fold bits");
-    if (e == -2) return fail(-17, "fold ratio must be 2^1 .. 2^5");
+    if (e == -2) return fail(-17, "fold ratio must be 2^1 .. 2^10");
     if (e) return fail(-100 - e, "fold launch failed");
     return 0;
 }
@@ -1109,6 +1176,7 @@ int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t d
 struct FriGraph {
     std::vector<uint64_t> key;
     hipGraphExec_t exec;
+    hipEvent_t done;  // recorded after every launch (see launch_cached)
 };
 static std::vector<FriGraph> g_fri_graphs;
 static std::mutex g_fri_mu;
@@ -1147,8 +1215,8 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
     if (!steps || nSteps == 0) return fail(-16, "no FRI steps");
     if (steps[0] != nBitsExt || nBitsExt > 32) return fail(-16, "steps[0] must equal nBitsExt (<= 32)");
     for (uint64_t s = 1; s < nSteps; s++)
-        if (steps[s] >= steps[s - 1] || steps[s - 1] - steps[s] > 5)
-            return fail(-17, "FRI steps must decrease by 1 .. 5 bits");
+        if (steps[s] >= steps[s - 1] || steps[s - 1] - steps[s] > 10)
+            return fail(-17, "FRI steps must decrease by 1 .. 10 bits");
     if (!d_pol || (nSteps > 1 && (!d_work || !d_challenges))) return fail(-13, "null buffer");
     if (nSteps > 1 && (!d_sources || !d_nodes)) return fail(-13, "null layer buffers");
     for (uint64_t s = 0; s + 1 < nSteps; s++)
@@ -1165,7 +1233,7 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
     std::lock_guard<std::mutex> lk(g_fri_mu);
     for (auto &g : g_fri_graphs)
         if (g.key == key) {
-            HIP_TRY(hipGraphLaunch(g.exec, st), "FRI graph launch");
+            HIP_TRY(launch_cached(g.exec, g.done, st), "FRI graph launch");
             return 0;
         }
     if (!g_fri_capture) HIP_TRY(hipStreamCreateWithFlags(&g_fri_capture, hipStreamNonBlocking), "FRI capture stream");
@@ -1183,19 +1251,25 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
     ee = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ee != hipSuccess) return fail(-100 - (int64_t)ee, std::string("FRI graph instantiate: ") + hipGetErrorString(ee));
-    if (g_fri_graphs.size() >= 8) {  // small cache: drop the oldest graph
-        (void)hipGraphExecDestroy(g_fri_graphs.front().exec);
+    hipEvent_t done = nullptr;
+    ee = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (ee != hipSuccess) {
+        (void)hipGraphExecDestroy(ex);
+        return fail(-100 - (int64_t)ee, "FRI graph event");
+    }
+    if (g_fri_graphs.size() >= 8) {  // small cache: drop the oldest graph once its last launch has ended
+        retire_exec(g_fri_graphs.front().exec, g_fri_graphs.front().done);
         g_fri_graphs.erase(g_fri_graphs.begin());
     }
-    g_fri_graphs.push_back({key, ex});
-    HIP_TRY(hipGraphLaunch(ex, st), "FRI graph launch");
+    g_fri_graphs.push_back({key, ex, done});
+    HIP_TRY(launch_cached(ex, done, st), "FRI graph launch");
     return 0;
 }
 
 void pil2hip_fri_release(void)
 {
     std::lock_guard<std::mutex> lk(g_fri_mu);
-    for (auto &g : g_fri_graphs) (void)hipGraphExecDestroy(g.exec);
+    for (auto &g : g_fri_graphs) retire_exec(g.exec, g.done);
     g_fri_graphs.clear();
 }
 
@@ -1262,53 +1336,87 @@ static int64_t write_file(const char *path, const void *root, const uint64_t *tr
     return ok ? 0 : fail(-21, std::string("short write to ") + path);
 }
 
-static int64_t write_custom_commit_impl(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
-                                        void *d_buffers, void *buffer, const char *bufferFile)
+// A host trace committed on the device, with the outputs the CPU backend leaves
+// in host memory: the root, and optionally the row-major LDE (NExt x nCols) and
+// the Merkle nodes. Synchronous on the d_buffers stream (or the null stream).
+static int64_t host_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                           void *d_buffers, const void *buffer, uint64_t *h_lde_rm, uint64_t *h_nodes,
+                           const char *who)
 {
     int64_t rc = check_family();
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
+    if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
     if (!root || (!buffer && nCols)) return fail(-13, "null buffer");
     uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
     uint64_t nTree = pil2hip_tree_num_elements(NExt, arity);
     hipStream_t st = d_buffers ? ((DeviceBuffersHip *)d_buffers)->stream : (hipStream_t)0;
-    bool want_file = bufferFile && bufferFile[0];
     uint64_t *d_trace = nullptr, *d_scratch = nullptr, *d_lde = nullptr, *d_nodes = nullptr, *d_rm = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(d_trace); (void)hipFree(d_scratch); (void)hipFree(d_lde); (void)hipFree(d_nodes); (void)hipFree(d_rm);
     };
     uint64_t tsz = N * nCols, lsz = NExt * nCols;
+    bool want_rm = h_lde_rm && lsz;
     if (hipMalloc(&d_trace, (tsz ? tsz : 1) * 8) != hipSuccess || hipMalloc(&d_scratch, (tsz ? tsz : 1) * 8) != hipSuccess ||
-        hipMalloc(&d_lde, (lsz ? lsz : 1) * 8) != hipSuccess || hipMalloc(&d_nodes, nTree * 8) != hipSuccess) {
+        hipMalloc(&d_lde, (lsz ? lsz : 1) * 8) != hipSuccess || hipMalloc(&d_nodes, nTree * 8) != hipSuccess ||
+        (want_rm && hipMalloc(&d_rm, lsz * 8) != hipSuccess)) {
         cleanup();
-        return fail(-22, "write_custom_commit: device allocation failed");
+        return fail(-22, std::string(who) + ": device allocation failed");
     }
     if (tsz && hipMemcpyAsync(d_trace, buffer, tsz * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
         cleanup();
-        return fail(-23, "write_custom_commit: H2D copy failed");
+        return fail(-23, std::string(who) + ": H2D copy failed");
     }
     rc = pil2hip_lde(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st);
     if (!rc) rc = pil2hip_merkelize(d_nodes, d_lde, NExt, nCols, 1, NExt, arity, st);
     if (!rc) rc = pil2hip_get_root((uint64_t *)root, d_nodes, NExt, arity, st);
-    if (!rc && want_file) {
-        std::vector<uint64_t> lde(lsz), nodes(nTree);
-        if (lsz) {
-            if (hipMalloc(&d_rm, lsz * 8) != hipSuccess) {
-                cleanup();
-                return fail(-22, "write_custom_commit: device allocation failed");
-            }
-            rc = pil2hip_transpose_cm2rm(d_rm, d_lde, NExt, nCols, NExt, nCols, st);
-            if (!rc && hipMemcpyAsync(lde.data(), d_rm, lsz * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
-                rc = fail(-24, "D2H copy failed");
-        }
-        if (!rc && hipMemcpyAsync(nodes.data(), d_nodes, nTree * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
-            rc = fail(-24, "D2H copy failed");
-        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(-25, "stream sync failed");
-        if (!rc) rc = write_file(bufferFile, root, (const uint64_t *)buffer, tsz, lde.data(), lsz, nodes.data(), nTree);
+    if (!rc && want_rm) {
+        rc = pil2hip_transpose_cm2rm(d_rm, d_lde, NExt, nCols, NExt, nCols, st);
+        if (!rc && hipMemcpyAsync(h_lde_rm, d_rm, lsz * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = fail(-24, std::string(who) + ": D2H copy failed");
     }
-    (void)hipStreamSynchronize(st);  // also after an error: the buffers are freed next
-    cleanup();
+    if (!rc && h_nodes && hipMemcpyAsync(h_nodes, d_nodes, nTree * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = fail(-24, std::string(who) + ": D2H copy failed");
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(-25, "stream sync failed");
+    cleanup();  // also after an error: the stream has drained
     return rc;
+}
+
+static int64_t write_custom_commit_impl(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                                        void *d_buffers, void *buffer, const char *bufferFile)
+{
+    bool want_file = bufferFile && bufferFile[0];
+    if (!want_file)
+        return host_commit(root, arity, nBits, nBitsExt, nCols, d_buffers, buffer, nullptr, nullptr,
+                           "write_custom_commit");
+    if ((nBitsExt < nBits) || nBitsExt > NTT_MAX_LOG || arity < 2 || arity > 4)  // sizes below must be sane
+        return host_commit(root, arity, nBits, nBitsExt, nCols, d_buffers, buffer, nullptr, nullptr,
+                           "write_custom_commit");
+    uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
+    std::vector<uint64_t> lde(NExt * nCols), nodes(pil2hip_tree_num_elements(NExt, arity));
+    int64_t rc = host_commit(root, arity, nBits, nBitsExt, nCols, d_buffers, buffer, lde.data(), nodes.data(),
+                             "write_custom_commit");
+    if (!rc)
+        rc = write_file(bufferFile, root, (const uint64_t *)buffer, N * nCols, lde.data(), lde.size(), nodes.data(),
+                        nodes.size());
+    return rc;
+}
+
+int64_t pil2hip_commit_witness(uint64_t *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols,
+                               const uint64_t *trace, uint64_t *aux_trace, uint64_t offset_dst, uint64_t offset_mt,
+                               void *d_buffers)
+{
+    if (!aux_trace) return fail(-13, "null aux_trace");
+    if (nBitsExt < nBits || nBitsExt > NTT_MAX_LOG || arity < 2 || arity > 4)
+        return host_commit(root, arity, nBits, nBitsExt, nCols, d_buffers, trace, nullptr, nullptr, "commit_witness");
+    uint64_t NExt = 1ull << nBitsExt, lsz = NExt * nCols, nTree = pil2hip_tree_num_elements(NExt, arity);
+    // the two sections must not overlap (the reference lays them out apart in aux_trace)
+    if (lsz && offset_dst < offset_mt + nTree && offset_mt < offset_dst + lsz)
+        return fail(-35, "commit_witness: the LDE and Merkle sections of aux_trace overlap");
+    return host_commit(root, arity, nBits, nBitsExt, nCols, d_buffers, trace, aux_trace + offset_dst,
+                       aux_trace + offset_mt, "commit_witness");
 }
 
 void write_custom_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *d_buffers,

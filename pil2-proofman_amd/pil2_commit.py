@@ -83,6 +83,9 @@ _SIGS = {
     "pil2hip_serve_job_bytes": (_u64, [_u64, _u64, _u64, _u64, _u64, _u64, ctypes.c_bool]),
     "pil2hip_serve_destroy": (None, [_vp]),
     "pil2hip_get_root": (_i64, [_vp, _vp, _u64, _u64, _vp]),
+    "pil2hip_level_words": (_u64, [_u64, _u64]),
+    "pil2hip_get_level": (_i64, [_vp, _vp, _u64, _u64, _u64, _vp]),
+    "pil2hip_commit_witness": (_i64, [_vp, _u64, _u64, _u64, _u64, _vp, _vp, _u64, _u64, _vp]),
     "pil2hip_fri_fold": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_fri_transpose": (_i64, [_vp, _vp, _u64, _u64, _vp]),
     "pil2hip_transpose_cm2rm": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
@@ -141,8 +144,17 @@ def to_host(t):
     return t.detach().cpu().numpy().view(np.uint64)
 
 
-def empty(n, device="cuda"):
-    return _torch().empty(int(n), dtype=_torch().int64, device=device)
+def empty(n, device="cuda", stream=None):
+    """Device int64 tensor of n words. With `stream` (a torch stream) the block is
+    allocated in that stream's context, so the caching allocator hands it out again
+    only after the work queued on `stream` has used it: temporaries that this module
+    allocates for an asynchronous call on a caller's stream stay valid until the
+    call's kernels have run."""
+    torch = _torch()
+    if stream is None:
+        return torch.empty(int(n), dtype=torch.int64, device=device)
+    with torch.cuda.stream(stream):
+        return torch.empty(int(n), dtype=torch.int64, device=device)
 
 
 # ------------------------------------------------------------------ ABI --
@@ -169,6 +181,23 @@ def write_custom_commit(root, arity, n_bits, n_bits_ext, n_cols, d_buffers, buff
     err = _lib.pil2hip_last_error().decode()
     if err:
         raise HipError("write_custom_commit: " + err)
+
+
+def commit_witness(root, arity, n_bits, n_bits_ext, n_cols, trace, aux_trace, offset_dst, offset_mt, d_buffers=None):
+    """commit_witness_cpu's data path (starks_api.cpp:683-723) through
+    pil2hip_commit_witness: trace (host uint64, row-major 2^n_bits x n_cols) ->
+    aux_trace[offset_dst:] = row-major LDE, aux_trace[offset_mt:] = Merkle nodes,
+    root (4 words). aux_trace and root are host uint64 arrays written in place."""
+    tr = np.ascontiguousarray(trace, dtype=np.uint64)
+    for a in (root, aux_trace):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]):
+            raise ValueError("commit_witness: root and aux_trace must be contiguous uint64 arrays")
+    ne = 1 << n_bits_ext
+    if root.size < 4 or aux_trace.size < max(offset_dst + ne * n_cols, offset_mt + get_tree_num_elements(ne, arity)):
+        raise ValueError("commit_witness: root or aux_trace too small")
+    _check(_lib.pil2hip_commit_witness(root.ctypes.data_as(_vp), arity, n_bits, n_bits_ext, n_cols,
+                                       tr.ctypes.data_as(_vp), aux_trace.ctypes.data_as(_vp), offset_dst, offset_mt,
+                                       d_buffers))
 
 
 def const_tree_size(n_bits_ext, n_constants, arity):
@@ -240,7 +269,7 @@ class NTT_Goldilocks:
         need = 0 if lg < 4 else (2 * size * ncols if row_major else (size * ncols if dst.data_ptr() == src.data_ptr()
                                                                        else 0))
         if need and (buffer is None or buffer.numel() < need):
-            buffer = empty(need, device=src.device)
+            buffer = empty(need, device=src.device, stream=stream)
         _check(_lib.pil2hip_ntt(_ptr(dst), _ptr(src), _ptr(buffer) if need else None, lg, ncols, bool(inverse),
                                 bool(row_major), _stream(stream)))
 
@@ -254,7 +283,7 @@ class NTT_Goldilocks:
         if n == 0 or ncols == 0:
             return
         if buffer is None:
-            buffer = empty(n * ncols, device=input.device)
+            buffer = empty(n * ncols, device=input.device, stream=stream)
         _check(_lib.pil2hip_lde(_ptr(output), _ptr(input), _ptr(buffer), self._log2(n),
                                 self._log2(n_extended), ncols, _stream(stream)))
 
@@ -291,7 +320,7 @@ class MerkleTreeGL:
     def merkelize(self, stream=None):
         """merkleTreeGL.cpp:316-336"""
         if self.nodes is None:
-            self.nodes = empty(self.num_nodes, device=self.source.device)
+            self.nodes = empty(self.num_nodes, device=self.source.device, stream=stream)
         _check(_lib.pil2hip_merkelize(_ptr(self.nodes), _ptr(self.source), self.height, self.width,
                                       self.row_stride, self.col_stride, self.arity, _stream(stream)))
 
@@ -309,19 +338,14 @@ class MerkleTreeGL:
             return int(math.ceil(math.log2(self.height) / math.log2(self.arity))) - self.last_level_verification
         return 0
 
-    def get_level(self):
-        """merkleTreeGL.cpp:84-101: the nodes of the level with <= arity^llv entries,
-        zero-padded to arity^llv digests."""
+    def get_level(self, stream=None):
+        """merkleTreeGL.cpp:84-101 (pil2hip_get_level): the nodes of the level with
+        <= arity^llv entries, zero-padded to arity^llv digests; empty for llv 0."""
         llv = self.last_level_verification
-        if llv == 0:
-            return np.zeros(0, dtype=np.uint64)
-        nodes = to_host(self.nodes)
-        n, off, lim = self.height, 0, self.arity ** llv
-        while n > lim:
-            n = (n + self.arity - 1) // self.arity
-            off += n * self.arity * HASH_SIZE
-        out = np.zeros(lim * HASH_SIZE, dtype=np.uint64)
-        out[:n * HASH_SIZE] = nodes[off:off + n * HASH_SIZE]
+        out = np.zeros(int(_lib.pil2hip_level_words(self.arity, llv)), dtype=np.uint64)
+        if out.size:
+            _check(_lib.pil2hip_get_level(out.ctypes.data_as(_vp), _ptr(self.nodes), self.height, self.arity, llv,
+                                          _stream(stream)))
         return out
 
     def get_group_proof(self, idx):
@@ -380,9 +404,9 @@ class FRI:
         for s_ in range(n - 1):
             t = trees[s_]
             if t.source is None:
-                t.source = empty(3 << steps[s_], device=pol.device)
+                t.source = empty(3 << steps[s_], device=pol.device, stream=stream)
             if t.nodes is None:
-                t.nodes = empty(t.num_nodes, device=pol.device)
+                t.nodes = empty(t.num_nodes, device=pol.device, stream=stream)
             t.row_stride, t.col_stride = t.width, 1
         st = (ctypes.c_uint64 * n)(*steps)
         srcs = (_vp * max(n - 1, 1))(*[_ptr(t.source) for t in trees[:n - 1]])
@@ -406,7 +430,7 @@ class FRI:
         """fri.hpp:102-109: regroup (getTransposed) into tree.source, merkelize, root."""
         degree = 1 << current_bits
         if tree.source is None:
-            tree.source = empty(degree * 3, device=pol.device)
+            tree.source = empty(degree * 3, device=pol.device, stream=stream)
         _check(_lib.pil2hip_fri_transpose(_ptr(tree.source), _ptr(pol), degree, next_bits, _stream(stream)))
         tree.row_stride, tree.col_stride = tree.width, 1
         tree.merkelize(stream)
@@ -440,10 +464,10 @@ def commit_host(h_trace, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None
     if numel < (1 << n_bits) * n_cols:
         raise ValueError("commit_host: trace smaller than 2^n_bits * n_cols")
     dev = torch.device("cuda", torch.cuda.current_device())
-    lde = empty(max(ne * n_cols, 1), dev) if lde is None else lde
-    nodes = empty(get_tree_num_elements(ne, arity), dev) if nodes is None else nodes
+    lde = empty(max(ne * n_cols, 1), dev, stream) if lde is None else lde
+    nodes = empty(get_tree_num_elements(ne, arity), dev, stream) if nodes is None else nodes
     words = _lib.pil2hip_commit_host_stage_words(n_bits, n_cols, arity, group_cols)
-    stage = empty(max(words, 1), dev) if stage is None else stage
+    stage = empty(max(words, 1), dev, stream) if stage is None else stage
     if stage.numel() < words:
         raise ValueError("commit_host: stage smaller than %d words" % words)
     _check(_lib.pil2hip_commit_host(_ptr(lde), _ptr(nodes), ctypes.c_void_p(ptr), _ptr(stage), arity, n_bits,
@@ -471,9 +495,9 @@ def commit(trace_dev, arity, n_bits, n_bits_ext, n_cols, lde=None, nodes=None, s
     Returns (lde column-major, nodes)."""
     n, ne = 1 << n_bits, 1 << n_bits_ext
     dev = trace_dev.device
-    lde = empty(ne * n_cols, dev) if lde is None else lde
-    nodes = empty(get_tree_num_elements(ne, arity), dev) if nodes is None else nodes
-    scratch = empty(max(n * n_cols, 1), dev) if scratch is None else scratch
+    lde = empty(ne * n_cols, dev, stream) if lde is None else lde
+    nodes = empty(get_tree_num_elements(ne, arity), dev, stream) if nodes is None else nodes
+    scratch = empty(max(n * n_cols, 1), dev, stream) if scratch is None else scratch
     _check(_lib.pil2hip_commit(_ptr(lde), _ptr(nodes), _ptr(trace_dev), _ptr(scratch), arity, n_bits,
                                n_bits_ext, n_cols, _stream(stream)))
     return lde, nodes
@@ -486,10 +510,10 @@ def compute_q(q_dev, n_bits, n_bits_ext, q_deg, q_dim=3, arity=3, row_major=True
     row_major=False). Returns (cmQ column-major q_deg*q_dim x NExt, nodes or None)."""
     n, ne = 1 << n_bits, 1 << n_bits_ext
     dev = q_dev.device
-    cmq = empty(ne * q_deg * q_dim, dev) if cmq is None else cmq
+    cmq = empty(ne * q_deg * q_dim, dev, stream) if cmq is None else cmq
     if merkelize and nodes is None:
-        nodes = empty(get_tree_num_elements(ne, arity), dev)
-    scratch = empty(max(n * q_deg * q_dim, 1), dev) if scratch is None else scratch
+        nodes = empty(get_tree_num_elements(ne, arity), dev, stream)
+    scratch = empty(max(n * q_deg * q_dim, 1), dev, stream) if scratch is None else scratch
     rs, cs = (q_dim, 1) if row_major else (1, ne)
     _check(_lib.pil2hip_compute_q(_ptr(cmq), _ptr(nodes) if merkelize else None, _ptr(q_dev), rs, cs,
                                   _ptr(scratch), n_bits, n_bits_ext, q_deg, q_dim, arity, _stream(stream)))
@@ -589,7 +613,7 @@ def group_proofs(src, n_rows, n_cols, nodes, arity, llv, idx, row_stride=1, col_
     n_rows). Returns a device tensor of len(idx) * proof_words(...) words."""
     pw = proof_words(n_rows, n_cols, arity, llv)
     cs = n_rows if col_stride is None else col_stride
-    out = empty(max(idx.numel() * pw, 1), src.device)
+    out = empty(max(idx.numel() * pw, 1), src.device, stream)
     _check(_lib.pil2hip_group_proofs(_ptr(out), _ptr(src), row_stride, cs, n_cols, _ptr(nodes), n_rows, arity, llv,
                                      _ptr(idx), idx.numel(), _stream(stream)))
     return out
