@@ -139,6 +139,20 @@ int64_t pil2hip_merkelize(uint64_t *d_nodes, const uint64_t *d_src, uint64_t nRo
 int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t arity,
                        uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
 
+/* No counterpart in the reference. pil2hip_commit without d_scratch, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): nothing is allocated, by this call or
+ * any call before it. The scratch is a device array inside the library of
+ * PIL2HIP_NOALLOC_SCRATCH_WORDS words (128 MiB), one per GPU, held from the
+ * library's load until the process ends; 2^nBits * nCols above that returns -36.
+ * All calls on one GPU share that array: calls that overlap on one GPU (on
+ * different streams, or from different threads without ordering) race and give
+ * wrong LDEs and roots. Calls ordered on one stream are safe. Outputs as
+ * pil2hip_commit's (d_lde column-major, d_nodes), on `stream`. */
+#define PIL2HIP_NOALLOC_SCRATCH_WORDS (1ull << 24)
+int64_t pil2hip_commit_noalloc(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t arity,
+                               uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream);
+
 /* pil2hip_commit for a trace in host memory, with the transfer overlapped (no
  * reference counterpart with this signature: it is the column-chunk absorb of the
  * reference's streaming commit, stream_commit.cuh:9-20 / commit_witness_streaming,

@@ -54,8 +54,14 @@ static const uint64_t kRoots[33] = {
     2530564950562219707ULL, 1905180297017055339ULL, 3524815499551269279ULL,
     7277203076849721926ULL};  // goldilocks_base_field.cpp:11-44
 
-// Per-device twiddle tables (built once per device, kept until process exit).
+// Per-device twiddle tables (built once per device, kept until process exit). They
+// live in a device array of the code object rather than in hipMalloc'd memory, so
+// that no entry point allocates for them: each device holds its own copy, found by
+// hipGetSymbolAddress with that device current.
 static const int kMaxDevices = 16;
+static const size_t kTabRoots = (size_t)2 << NTT_LOG_TILE, kTabTw = 4096;
+static const size_t kTabWords = 2 * kTabRoots + 6 * kTabTw;
+static __device__ __align__(256) uint64_t g_tab_dev[kTabWords];
 static NttTablesHost g_tables[kMaxDevices];
 static bool g_tables_ready[kMaxDevices];
 
@@ -66,8 +72,8 @@ static int64_t ensure_tables(NttTablesHost **out)
     if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_tables_ready[dev]) {
-        const size_t nr = (size_t)2 << NTT_LOG_TILE, nt = 4096;
-        std::vector<uint64_t> h(2 * nr + 6 * nt);
+        const size_t nr = kTabRoots, nt = kTabTw;
+        std::vector<uint64_t> h(kTabWords);
         uint64_t *rf = h.data(), *ri = rf + nr, *t = ri + nr;
         for (uint32_t k = 0; k <= NTT_LOG_TILE; k++) {  // rf[2^k + y] = w_{2^k}^y, ri[2^k + y] = w_{2^k}^-y
             uint64_t w = kRoots[k], wi = gl_inv(w), a = 1, b = 1;
@@ -93,7 +99,7 @@ static int64_t ensure_tables(NttTablesHost **out)
             e = gl_mul(e, 7); f = gl_mul(f, s4096);
         }
         uint64_t *dptr = nullptr;
-        HIP_TRY(hipMalloc(&dptr, h.size() * 8), "hipMalloc(tables)");
+        HIP_TRY(hipGetSymbolAddress((void **)&dptr, HIP_SYMBOL(g_tab_dev)), "hipGetSymbolAddress(tables)");
         HIP_TRY(hipMemcpy(dptr, h.data(), h.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(tables)");
         NttTablesHost &T = g_tables[dev];
         T.tkf = dptr;
@@ -300,6 +306,45 @@ int64_t pil2hip_commit(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_tra
 {
     return commit_impl(d_lde, d_nodes, d_trace, d_scratch, arity, nBits, nBitsExt, nCols, (hipStream_t)stream,
                        timing_events());
+}
+
+// ------------------------------------------------- no-allocation commit --
+// pil2hip_commit for callers that cannot allocate (graph capture, tight serving
+// loops): the column-major trace scratch is this file-scope device array, of
+// PIL2HIP_NOALLOC_SCRATCH_WORDS words per device, held until the process ends.
+// In host code `scratch` names a host-side stand-in, never a device address: the
+// address of each device's copy comes from hipGetSymbolAddress with that device
+// current, once per device, and only that pointer reaches kernels.
+static __device__ __align__(256) uint64_t scratch[PIL2HIP_NOALLOC_SCRATCH_WORDS];
+static uint64_t *g_scratch_ptr[kMaxDevices];
+
+static int64_t noalloc_scratch(uint64_t **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_scratch_ptr[dev]) {
+        uint64_t *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+int64_t pil2hip_commit_noalloc(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d_trace, uint64_t arity,
+                               uint64_t nBits, uint64_t nBitsExt, uint64_t nCols, void *stream)
+{
+    if (nBits > 63 || (nCols && ((1ull << nBits) > PIL2HIP_NOALLOC_SCRATCH_WORDS / nCols)))
+        return fail(-36, "pil2hip_commit_noalloc: 2^nBits * nCols exceeds PIL2HIP_NOALLOC_SCRATCH_WORDS");
+    // the twiddle tables (a device array too) and the scratch address, once per device
+    NttTablesHost *tb;
+    int64_t rc = ensure_tables(&tb);
+    if (rc) return rc;
+    uint64_t *s = nullptr;
+    if ((rc = noalloc_scratch(&s))) return rc;
+    return commit_impl(d_lde, d_nodes, d_trace, s, arity, nBits, nBitsExt, nCols, (hipStream_t)stream, nullptr);
 }
 
 // ------------------------------------------------------- streamed commit --

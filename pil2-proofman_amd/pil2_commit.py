@@ -48,6 +48,7 @@ _SIGS = {
     "pil2hip_lde": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _vp]),
     "pil2hip_merkelize": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_commit": (_i64, [_vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
+    "pil2hip_commit_noalloc": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_const_tree_size": (_u64, [_u64, _u64, _u64]),
     "pil2hip_const_tree": (_i64, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _vp]),
     "pil2hip_calculate_const_tree": (_i64, [_vp, _vp, _u64, _u64, _u64, _u64, _vp]),
