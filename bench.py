@@ -213,6 +213,42 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
     }
 
 
+LDE_KERNELS = ("k_transpose_rm2cm", "k_ntt_strided_inv", "k_lde_block<false, false>", "k_ntt_strided_fwd<false>")
+# single-rate VALU instructions of one radix-2 butterfly at the floor: a reduced
+# Goldilocks product (4 v_mad_u64_u32 + 9-instruction reduction) plus a 6-instruction
+# add and a 6-instruction sub, canonical in and out
+INSTR_PER_BUTTERFLY = 24
+
+
+def _lde_valu_roofline(nb, nbe, cols, lde_ms, config):
+    """VALU roofline of the LDE: the algorithmic radix-2 butterflies (the N-point
+    inverse transform, then the NExt-point forward transform of the zero-extended
+    coefficients, whose first log2(B) stages only copy) against the rate at which
+    the VALU issues them at INSTR_PER_BUTTERFLY single-rate instructions each; plus
+    the measured share of VALU issue slots the four kernels fill (committed
+    counters, live time)."""
+    n, ne = 1 << nb, 1 << nbe
+    bfly = cols * (n // 2 * nb + ne // 2 * nb)
+    peak = VALU_PEAK_TOPS / INSTR_PER_BUTTERFLY  # T butterflies/s
+    achieved = bfly / (lde_ms / 1e3) / 1e12
+    out = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": "Tbutterfly/s",
+           "frac": round(achieved / peak, 4), "butterflies": bfly,
+           "floor_ms": round(bfly * INSTR_PER_BUTTERFLY / (VALU_PEAK_TOPS * 1e12) * 1e3, 3),
+           "model": "radix-2 butterflies of the iNTT (N/2 log N) and the forward NTT after zero extension "
+                    "(NExt/2 log N), %d single-rate instructions each at %.1f T lane-op/s"
+                    % (INSTR_PER_BUTTERFLY, VALU_PEAK_TOPS)}
+    cyc, lanes = 0.0, 0.0
+    for k in LDE_KERNELS:
+        c = _profile_kernel(k, config)
+        if "valu_wave_instr_per_launch" not in c:
+            return out
+        cyc += 4.0 * (c["valu_wave_instr_per_launch"] - c["valu_dual_issue_quad_cycles_per_launch"]) / N_SIMD
+        lanes += 64.0 * c["valu_wave_instr_per_launch"]
+    out["valu_issue_frac"] = round(cyc / (CLOCK_HZ * lde_ms / 1e3), 4)
+    out["lane_instructions"] = lanes
+    return out
+
+
 def _with_issue(roofline, config):
     """Adds the leaf kernel's measured VALU issue share to its roofline object."""
     f = _valu_issue(roofline["kernel"], config, roofline["avg_ms"])
@@ -224,11 +260,24 @@ def _with_issue(roofline, config):
     return roofline
 
 
-def run_batched(args, cfg):
+def _reference_fixture():
+    """The compiled reference's expectations at the configurations' real sizes
+    (tests/golden/config_reference.json), or None where the file is absent."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "config_reference.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def run_batched(args, cfg, embedded=False):
     """Mixed-size batch: all instances resident in HBM, committed (a) one by one
     through the plain per-instance path (commit + root back to the host, the way
     proofman calls commit_witness per instance) and (b) as one hipGraph launch of
-    pil2hip_commit_batch. Both produce every instance's root; they must agree."""
+    pil2hip_commit_batch. Both produce every instance's root; they must agree, and
+    equal the compiled reference's roots where the fixture file is present.
+    embedded: return a summary (at most 3 timed graph replays, no CPU baseline)
+    for the default benchmark line instead of printing a line of its own."""
     import ctypes
     import torch
     import datagen
@@ -294,17 +343,37 @@ def run_batched(args, cfg):
         p = pc.batch_shape_plan(j["n_bits"], j["n_bits_ext"], j["n_cols"], arity)[1]
         k = "tile 2^%d radix %d coop %d" % (p["ntt_tile_log"], 1 << p["ntt_radix_log"], p["coop_max_parents"])
         shape_plans[k] = shape_plans.get(k, 0) + 1
-    for _ in range(args.warmup):
+    b_steps, b_warm = (min(args.steps, 3), min(args.warmup, 1)) if embedded else (args.steps, args.warmup)
+    for _ in range(b_warm):
         batch.launch(plan=request)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(b_steps):
         batch.launch(plan=request)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    value = elems / (dt / args.steps) / 1e9
+    value = elems / (dt / b_steps) / 1e9
     roots = batch.roots_host()
     match = all(list(map(int, roots[k])) == list(map(int, plain_roots[k])) for k in range(len(shapes)))
+    fx = _reference_fixture()
+    ref_match = None
+    if fx is not None and fx["batched"]["seed"] == cfg["seed"] and len(fx["batched"]["roots"]) == len(shapes):
+        ref_match = all([int(v) for v in roots[k]] == [int(x, 16) for x in fx["batched"]["roots"][k]]
+                        for k in range(len(shapes)))
+    del batch, jobs
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if embedded:
+        out = {"workload": "64 instances, log2 rows U{12..21}, cols U{8..256}, blowup 2/4 (seed 0x%X), one "
+                           "hipGraph launch" % cfg["seed"],
+               "value": round(value, 4), "unit": "GElem/s", "ms_per_step": round(dt * 1e3 / b_steps, 3),
+               "steps": b_steps, "lde_elements": elems, "graph_branches": streams, "autotune_s": round(t_tune, 3),
+               "plain_value": round(plain_value, 4), "plain_ms_per_pass": round(dt_plain * 1e3, 3),
+               "batched_over_plain": round(value / plain_value, 4), "roots_match_plain": match,
+               "roots_match_reference": ref_match}
+        if not match or ref_match is False:
+            out["error"] = "batched roots differ from the per-instance or reference roots"
+        return out
 
     rate = width - 4
     perms = sum((1 << j["n_bits_ext"]) * ((j["n_cols"] + rate - 1) // rate) for j in shapes)
@@ -326,7 +395,7 @@ def run_batched(args, cfg):
                    "hbm_resident_GB": round(need / 1e9, 2), "path": "batched (pil2hip_commit_batch)"},
         "batched": {"graph_branches": streams, "plan": plan, "autotune_s": round(t_tune, 3),
                     "instances_per_shape_plan": shape_plans if not args.plan else None,
-                    "roots_match_plain": match},
+                    "roots_match_plain": match, "roots_match_reference": ref_match},
         "plain": {"value": round(plain_value, 4), "unit": "GElem/s", "ms_per_pass": round(dt_plain * 1e3, 3),
                   "path": "per-instance pil2hip_commit + root to host, one instance at a time",
                   "phases_ms_sum": dict(zip(["transpose", "intt_strided", "lde_block", "ntt_strided", "leaves",
@@ -339,12 +408,12 @@ def run_batched(args, cfg):
                          "traffic": None, "avg_ms": lde_ms / len(shapes)},
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
-    if not match:
-        out["error"] = "batched roots differ from the per-instance roots"
+    if not match or ref_match is False:
+        out["error"] = "batched roots differ from the per-instance or reference roots"
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched(shapes, arity, args.cpu_seconds)
     print(json.dumps(out), flush=True)
-    if not match:
+    if not match or ref_match is False:
         sys.exit(1)
 
 
@@ -484,20 +553,25 @@ def run_fri(args, cfg):
     print(json.dumps(out), flush=True)
 
 
-def run_serving(args, cfg):
+def run_serving(args, cfg, embedded=False):
     """Serving: jobs arrive as a Poisson stream and are admitted by the commit
     server (pil2hip_serve_*) under an HBM budget while earlier jobs still run;
     each job commits its trace (synthetic values generated in HBM from its seed)
     and returns its root and 32 group proofs (getGroupProof, llv 2). The arrival
     rate is 1.2x the throughput the same server sustains on a closed burst of
-    jobs. Reports aggregate GElem/s and p50/p99 of completion - arrival."""
+    jobs. Reports aggregate GElem/s and p50/p99 of completion - arrival. The jobs
+    of the stream's prefix that the fixture file holds are checked against the
+    compiled reference's roots and proof digests as they complete.
+    embedded: a bounded stream (the first 400 jobs, then the first 200 at 0.7x)
+    summarised for the default benchmark line, no CPU baseline."""
+    import hashlib
     import numpy as np
     import torch
     import datagen
     import pil2_commit as pc
 
     arity, llv, nq = cfg["arity"], cfg["llv"], cfg["queries"]
-    n_jobs = args.jobs or cfg["n_jobs"]
+    n_jobs = args.jobs or (400 if embedded else cfg["n_jobs"])
     shapes = datagen.batched_workload(seed=cfg["seed"], n_instances=n_jobs, min_log=12, max_log=22)
     qwords = datagen.raw_words(n_jobs * nq, cfg["seed"] ^ 0xA5A5A5A5).reshape(n_jobs, nq)
     pc.set_hash_family(2)
@@ -506,6 +580,20 @@ def run_serving(args, cfg):
     eng = pc.ServeEngine(budget, n_streams=args.serve_streams, max_inflight=args.serve_inflight, arity=arity,
                          llv=llv)
     elems = [(1 << j["n_bits_ext"]) * j["n_cols"] for j in shapes]
+    fx = _reference_fixture()
+    fx_jobs = fx["serving"]["jobs"] if fx is not None and fx["serving"]["seed"] == cfg["seed"] else []
+    checked = {"jobs": 0, "mismatch": 0}
+
+    def check(rec):
+        """Root and proof digest of a fixture job against the compiled reference's."""
+        k = rec["tag"]
+        if rec["status"] != 0 or k >= len(fx_jobs):
+            return
+        fj = fx_jobs[k]
+        got = hashlib.sha256(rec["proofs"][:nq * rec["pw"]].astype("<u8").tobytes()).hexdigest()
+        ok = [int(v) for v in rec["root"]] == [int(x, 16) for x in fj["root"]] and got == fj["proofs_sha256"]
+        checked["jobs"] += 1
+        checked["mismatch"] += 0 if ok else 1
 
     def submit(k):
         j = shapes[k]
@@ -521,6 +609,7 @@ def run_serving(args, cfg):
     while got < n_cal:
         for rec in eng.poll(max_n=256, timeout_ms=200):
             assert rec["status"] == 0, rec["status"]
+            check(rec)
             eng.forget(rec["job_id"])
             got += 1
     cal_s = time.perf_counter() - t0
@@ -546,6 +635,7 @@ def run_serving(args, cfg):
             for rec in eng.poll(max_n=256, timeout_ms=min(wait_ms, 50) if k_ < len(jobs) else 100):
                 if rec["status"] != 0:
                     bad_ += 1
+                check(rec)
                 lat_.append((rec["t_done_us"] - rec["t_submit_us"]) / 1e3)
                 qd_.append((rec["t_admit_us"] - rec["t_submit_us"]) / 1e3)
                 t_first_ = rec["t_submit_us"] if t_first_ is None else min(t_first_, rec["t_submit_us"])
@@ -561,12 +651,26 @@ def run_serving(args, cfg):
     lat, qdelay, span_s, bad, lam = open_loop(list(range(n_jobs)), cfg["load"], cfg["seed"])
     # the same server below capacity: the first jobs of the stream again at 0.7x, where
     # latency is the service time plus little queueing
-    n_low = min(400, n_jobs)
+    n_low = min(200 if embedded else 400, n_jobs)
     lat_lo, qd_lo, span_lo, bad_lo, lam_lo = open_loop(list(range(n_low)), 0.7, cfg["seed"] + 1)
     bad += bad_lo
     st = eng.stats()
     eng.close()
     value = sum(elems) / span_s / 1e9
+    bad += checked["mismatch"]
+    lat_ms = {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)}
+    lat_lo_ms = {"p50": round(float(np.percentile(lat_lo, 50)), 3), "p99": round(float(np.percentile(lat_lo, 99)), 3)}
+    if embedded:
+        return {"workload": "first %d jobs of the serving stream (log2 rows U{12..22}, cols U{8..256}, blowup 2/4, "
+                            "seed 0x%X), HBM budget %.0f GB, %d group proofs per job (llv %d)"
+                            % (n_jobs, cfg["seed"], budget / 1e9, nq, llv),
+                "offered_load": cfg["load"], "value": round(value, 4), "unit": "GElem/s",
+                "latency_ms": lat_ms, "arrival_rate_jobs_s": round(lam, 3),
+                "below_capacity": {"offered_load": 0.7, "jobs": n_low, "latency_ms": lat_lo_ms,
+                                   "GElem_s": round(sum(elems[:n_low]) / span_lo / 1e9, 4)},
+                "capacity_GElem_s": round(capacity / 1e9, 4), "failed_jobs": bad,
+                "reference_checked_jobs": checked["jobs"], "reference_mismatches": checked["mismatch"],
+                "peak_running": st["peak_running"]}
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(span_s * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -590,6 +694,7 @@ def run_serving(args, cfg):
                            "GElem_s": round(sum(elems[:n_low]) / span_lo / 1e9, 4)},
         "capacity_GElem_s": round(capacity / 1e9, 4), "arrival_rate_jobs_s": round(lam, 3),
         "lde_elements": int(sum(elems)), "failed_jobs": bad,
+        "reference_checked_jobs": checked["jobs"], "reference_mismatches": checked["mismatch"],
         "server": {k2: st[k2] for k2 in ("admitted", "completed", "failed", "peak_bytes", "peak_running")},
         "device": torch.cuda.get_device_name(0), "host": platform.node(),
     }
@@ -769,10 +874,10 @@ def _plain_from_host(args, pc, trace, lde, nodes, scratch, root, nb, nbe, cols, 
     pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(h2d_steps):
+    for _ in range(args.steps):  # as many steps as the HBM-resident timing
         pc.commit_host(host_trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, stage=stage)
         root_s = pc.MerkleTreeGL(arity, 0, True, ne, cols, nodes=nodes).get_root()
-    streamed_dt = (time.perf_counter() - t0) / h2d_steps
+    streamed_dt = (time.perf_counter() - t0) / args.steps
     assert np.array_equal(root_s, root), "streamed commit from host memory gave another root"
     del host_trace, stage
     with_h2d = {"value": round(ne * cols / h2d_dt / 1e9, 4), "unit": "GElem/s", "ms_per_step": round(h2d_dt * 1e3, 3),
@@ -780,8 +885,10 @@ def _plain_from_host(args, pc, trace, lde, nodes, scratch, root, nb, nbe, cols, 
                 "h2d_GBs": round(8 * n * cols / copy_dt / 1e9, 2),
                 "note": "trace in pinned host memory: H2D copy + commit + root read per step",
                 "streamed": {"value": round(ne * cols / streamed_dt / 1e9, 4), "ms_per_step": round(streamed_dt * 1e3, 3),
-                             "note": "pil2hip_commit_host: column groups copied on a copy stream, overlapped with "
-                                     "the previous group's LDE and leaf absorb"}}
+                             "steps": args.steps,
+                             "note": "pil2hip_commit_host: from the first host-to-device copy to the root on the "
+                                     "host; column groups copied on a copy stream, overlapped with the previous "
+                                     "group's LDE and leaf absorb"}}
     return with_h2d
 
 
@@ -836,6 +943,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-h2d", action="store_true",
                     help="plain configs: skip the passes from host memory (the with_h2d object)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="plain_large: leave out the embedded batched and serving runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate duration of the bounded CPU-baseline sample")
     ap.add_argument("--batch-streams", type=int, default=0,
@@ -874,6 +983,33 @@ def main():
     if args.plan:
         run_plain_plan(args, cfg)
         return
+    out = run_plain(args, cfg)
+    if args.config == "plain_large" and not args.plain_only:
+        # the rest of the metric ("batched vs plain; p50/p99 latency") in the same line
+        torch.cuda.empty_cache()
+        b = run_batched(args, CONFIGS["batched"], embedded=True)
+        out["batched"] = b
+        out["batched_over_plain"] = b["batched_over_plain"]
+        print("bench: batched %.3f GElem/s (%.2fx the per-instance path)" % (b["value"], b["batched_over_plain"]),
+              file=sys.stderr, flush=True)
+        sv = run_serving(args, CONFIGS["serving"], embedded=True)
+        out["serving"] = sv
+        out["latency_ms"] = sv["latency_ms"]
+        print("bench: serving p50 %.1f ms p99 %.1f ms" % (sv["latency_ms"]["p50"], sv["latency_ms"]["p99"]),
+              file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+    if out.get("batched", {}).get("error") or out.get("serving", {}).get("failed_jobs"):
+        sys.exit(1)
+
+
+def run_plain(args, cfg):
+    """The plain commit (one instance per call) of `cfg`, HBM-resident inputs; the
+    same commit from pinned host memory; rooflines; the CPU baseline. Returns the
+    benchmark line as a dict, with every device buffer released."""
+    import ctypes
+    import torch
+    import pil2_commit as pc
+
     nb, nbe, cols, arity = cfg["n_bits"], cfg["n_bits"] + cfg["blowup_bits"], cfg["cols"], cfg["arity"]
     n, ne, width = 1 << nb, 1 << nbe, 4 * arity
     pc.set_hash_family(2)
@@ -924,13 +1060,19 @@ def main():
     lde_ms = sum(phase_ms[0:4])
     lde_bytes = 8 * cols * (n + ne)
     pass_bytes = 16 * cols * n * 2 + 8 * cols * (n + ne) + 16 * cols * ne  # each pass reads+writes its array
+    lde_traffic = None
+    tr_k = [_profile_traffic(k, args.config) for k in LDE_KERNELS]
+    if all(t is not None for t in tr_k):
+        lde_traffic = sum(tr_k)
     roofline_lde = {
         "bound": "hbm", "kernels": "transpose + strided iNTT + fused block + strided NTT",
         "achieved": lde_bytes / (lde_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": lde_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "per_pass_achieved": pass_bytes / (lde_ms / 1e3) / 1e9,
         "per_pass_frac": pass_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": None, "avg_ms": lde_ms,
+        "traffic": lde_traffic, "avg_ms": lde_ms,
+        "limited_by": "valu: the field arithmetic's instruction count, not HBM (the 'valu' object)",
+        "valu": _lde_valu_roofline(nb, nbe, cols, lde_ms, args.config),
     }
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "GElem/s", "n_gpus": 1, "steps": args.steps,
@@ -949,9 +1091,16 @@ def main():
     }
     if with_h2d is not None:
         out["with_h2d"] = with_h2d
+        out["value_from_host"] = with_h2d["streamed"]["value"]
+    fx = _reference_fixture()
+    if fx is not None and args.config == "plain_large" and fx["plain_large"]["seed"] == cfg["seed"]:
+        out["root_matches_reference"] = [int(v) for v in root] == [int(x, 16) for x in fx["plain_large"]["root"]]
+    del trace, lde, nodes, scratch
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

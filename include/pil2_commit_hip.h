@@ -124,7 +124,7 @@ uint64_t pil2hip_tree_num_elements(uint64_t nRows, uint64_t arity);
 
 /* NTT_Goldilocks::LDE (ntt_goldilocks.cpp:369-404) on device:
  * d_lde[c*NExt + j] = p_c(7 * w_NExt^j), p_c interpolating column c of d_trace.
- * d_scratch: >= 2^nBits * nCols words. nBitsExt - nBits >= 0, nBitsExt <= 24. */
+ * d_scratch: >= 2^nBits * nCols words. nBitsExt - nBits >= 0, nBitsExt <= 28. */
 int64_t pil2hip_lde(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt,
                     uint64_t nCols, void *stream);
 
@@ -183,7 +183,7 @@ uint64_t pil2hip_commit_host_stage_words(uint64_t nBits, uint64_t nCols, uint64_
  * r*nCols + c (the CPU layout); otherwise column c at c*2^nBits (the GPU layout).
  * d_dst == d_src or NULL transforms in place. d_scratch: 2 * 2^nBits * nCols words
  * for row-major data; 2^nBits * nCols for in-place column-major data; unused
- * otherwise and for 2^nBits < 16. nBits <= 24; nCols == 0 is a no-op. */
+ * otherwise and for 2^nBits < 16. nBits <= 28; nCols == 0 is a no-op. */
 int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch, uint64_t nBits, uint64_t nCols,
                     bool inverse, bool rowMajor, void *stream);
 
@@ -198,7 +198,7 @@ int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch,
  * modified and must not overlap d_cmQ. d_scratch: >= qDeg*qDim*2^nBits words.
  * With d_nodes != NULL the rows of d_cmQ are then merkelized (the step's
  * merkelize, starks.hpp:228-235); the root is read with pil2hip_get_root.
- * 1 <= qDeg <= 2^(nBitsExt - nBits), nBitsExt <= 24. */
+ * 1 <= qDeg <= 2^(nBitsExt - nBits), nBitsExt <= 28. */
 int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_q, uint64_t qRowStride,
                           uint64_t qColStride, uint64_t *d_scratch, uint64_t nBits, uint64_t nBitsExt, uint64_t qDeg,
                           uint64_t qDim, uint64_t arity, void *stream);

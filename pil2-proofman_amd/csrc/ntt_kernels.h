@@ -6,12 +6,20 @@
 #define NTT_THREADS 1024                   // max threads per block (16 values each)
 #define NTT_LOG_TILE 14                    // 2^14 values per block tile
 #define NTT_LDS_WORDS (16384 + 1024)       // tile + one pad word per 16 (139,264 bytes)
-#define NTT_MAX_LOG 24                     // largest transform handled (2^24 rows)
+#define NTT_MAX_LOG 28                     // largest transform handled (2^28 rows)
+#define NTT_MAX_STRIDED 10                 // most index bits one strided launch covers
+
+// Split tables of one domain 2^(2*bits): w^x = lo[x & (2^bits - 1)] * hi[x >> bits]
+// (x < 2^(2*bits), w = the domain's root, forward and inverse) and 7^e likewise.
+struct NttSplitTables {
+    uint64_t *twl_f, *twh_f, *twl_i, *twh_i;  // 2^bits entries each
+    uint64_t *shl, *shh;                   // 2^bits entries each
+    uint32_t bits;
+};
 
 struct NttTablesHost {
     uint64_t *tkf, *tki;                   // 2^15 entries each: [2^k + y] = w_{2^k}^(+-y), k <= 14
-    uint64_t *twl_f, *twh_f, *twl_i, *twh_i;  // 4096 entries each
-    uint64_t *shl, *shh;                   // 4096 entries each
+    NttSplitTables split[2];               // [0]: the 2^24 domain (bits 12), [1]: 2^28 (bits 14)
 };
 
 // LDE launch plan: the largest block tile (log2 values, <= 14; a shape that needs a

@@ -26,6 +26,8 @@
 // Each thread keeps 16 values per round; rounds exchange through LDS. The first
 // round of a pass reads global memory and the last one writes it, both in runs
 // of >= 64 bytes (whole 128-byte lines per XCD; see strided_tile).
+#include <algorithm>
+
 #include "gl_field.h"
 #include "ntt_kernels.h"
 
@@ -34,24 +36,30 @@
 //   tkf[2^k + y] = w_{2^k}^y, tki[2^k + y] = w_{2^k}^-y  (k <= 14, y < 2^k)  inside tiles:
 //        one compact table per round size, so a round's twiddles of 2^k <= 2^12
 //        entries fit the 32 KiB L1 instead of being spread over a 2^14-entry table
-//   twl/twh: w_{2^24}^x = lo[x & 4095] * hi[x >> 12]   (x < 2^24)    between passes
-//   shl/shh: 7^e = lo[e & 4095] * hi[e >> 12]                        coset shift
+//   twl/twh: w_D^x = lo[x & (2^bits - 1)] * hi[x >> bits]   (x < D)   between passes
+//   shl/shh: 7^e = lo[e & (2^bits - 1)] * hi[e >> bits]              coset shift
+// over the domain D = 2^(2*bits): bits = 12 (D = 2^24, lo tables of 32 KiB) for
+// transforms up to 2^24 points, bits = 14 (D = 2^28) above that.
 struct NttTables {
     const uint64_t *tkf, *tki;
     const uint64_t *twl_f, *twh_f, *twl_i, *twh_i;
     const uint64_t *shl, *shh;
+    uint32_t bits;
 };
+
+// the tables for transforms of at most 2^log_max points
+static NttTables tables_for(const NttTablesHost *th, uint32_t log_max)
+{
+    const NttSplitTables &S = th->split[log_max > 24 ? 1 : 0];
+    return {th->tkf, th->tki, S.twl_f, S.twh_f, S.twl_i, S.twh_i, S.shl, S.shh, S.bits};
+}
+
+__device__ __forceinline__ uint32_t tw_lo(uint32_t x, const NttTables &tb) { return x & ((1u << tb.bits) - 1u); }
+__device__ __forceinline__ uint32_t tw_hi(uint32_t x, const NttTables &tb) { return x >> tb.bits; }
 
 // w_16^k and w_16^-k for k < 8 (w_16 = W[4] = 2^12)
 __constant__ uint64_t c_w16f[8];
 __constant__ uint64_t c_w16i[8];
-
-__device__ __forceinline__ uint64_t tw_pow(const uint64_t *lo, const uint64_t *hi, uint32_t x)
-{
-    uint64_t a = lo[x & 4095u];
-    uint32_t h = x >> 12;
-    return h ? gl_mul(a, hi[h]) : a;
-}
 
 __device__ __forceinline__ uint32_t lds_addr(uint32_t x) { return x + (x >> 4); }
 
@@ -237,9 +245,18 @@ __device__ __forceinline__ uint32_t round_m(const RoundPos &p, uint32_t j, uint3
 //
 // DIF rounds (iNTT pass 1): index bits from the top, 2^r-point DIF, then value j
 // times w_{2^(q+r)}^-(o_lo * bitrev(j)).
-template <int R>
+// A tile that covers index bits [qlo, qlo + L) of m (SPLIT: a pass over more
+// bits than one tile holds, in two launches) takes its twiddles at the global
+// position: round bits q + qlo, lower index bits (o_lo << qlo) | m_lo.
+template <bool SPLIT>
+__device__ __forceinline__ uint32_t global_o(uint32_t o_lo, uint32_t qlo, uint32_t m_lo)
+{
+    return SPLIT ? (o_lo << qlo) | m_lo : o_lo;
+}
+
+template <int R, bool SPLIT = false>
 __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint32_t lt, uint32_t q,
-                                                  const NttTables &tb)
+                                                  const NttTables &tb, uint32_t qlo = 0, uint32_t m_lo = 0)
 {
     constexpr int LR = Log2<R>::v;
     constexpr int G = 16 / R;
@@ -251,10 +268,12 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
         dft_dif<R, true>(v);
-        if (p.o_lo) {
+        uint32_t og = global_o<SPLIT>(p.o_lo, qlo, m_lo);
+        if (og) {
             uint64_t w[R];
+            uint32_t base = 1u << (q + (SPLIT ? qlo : 0) + LR);
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = tb.tki[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
+            for (int j = 1; j < R; j++) w[j] = tb.tki[base + og * bitrev32(j, LR)];
             twiddle_rest<R>(v, w);
         }
 #pragma unroll
@@ -264,9 +283,9 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
 
 // DIT rounds (NTT pass 3): index bits from the bottom; value j times
 // w_{2^(q+r)}^(bitrev(j) * o_lo), then the 2^r-point DIT (INV: inverse roots).
-template <int R, bool INV = false>
+template <int R, bool INV = false, bool SPLIT = false>
 __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint32_t lt, uint32_t q,
-                                                  const NttTables &tb)
+                                                  const NttTables &tb, uint32_t qlo = 0, uint32_t m_lo = 0)
 {
     constexpr int LR = Log2<R>::v;
     constexpr int G = 16 / R;
@@ -277,10 +296,12 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
         uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
-        if (p.o_lo) {
+        uint32_t og = global_o<SPLIT>(p.o_lo, qlo, m_lo);
+        if (og) {
             uint64_t w[R];
+            uint32_t base = 1u << (q + (SPLIT ? qlo : 0) + LR);
 #pragma unroll
-            for (int j = 1; j < R; j++) w[j] = (INV ? tb.tki : tb.tkf)[(1u << (q + LR)) + p.o_lo * bitrev32(j, LR)];
+            for (int j = 1; j < R; j++) w[j] = (INV ? tb.tki : tb.tkf)[base + og * bitrev32(j, LR)];
             twiddle_rest<R>(v, w);
         }
         dft_dit<R, INV>(v);
@@ -307,18 +328,43 @@ __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uin
     __syncthreads();
 }
 
+// Which part of m (Ltot bits, global stride n2) a strided block covers: bits
+// [qlo, qlo + L) vary inside the tile; the lines i0 .. i0 + 2^lt - 1 and the other
+// bits of m come from the block index (line tile fastest). m_lo: bits below qlo,
+// m_hi: bits above qlo + L. The tile's element x sits at col + base + (x >> lt) *
+// stride + (x & (2^lt - 1)).
+struct StridedBlock {
+    uint64_t base, stride;
+    uint32_t m_lo, m_hi;
+};
+template <bool SPLIT>
+__device__ __forceinline__ StridedBlock strided_block(uint32_t lt, uint32_t log_n2, uint32_t qlo, uint32_t L)
+{
+    uint32_t bx = strided_tile(blockIdx.x, lt);
+    if (!SPLIT) return {(uint64_t)bx << lt, 1ull << log_n2, 0u, 0u};
+    uint32_t tiles_log = log_n2 - lt;
+    uint64_t i0 = (uint64_t)(bx & ((1u << tiles_log) - 1)) << lt;
+    uint32_t other = bx >> tiles_log;
+    uint32_t m_lo = other & ((1u << qlo) - 1), m_hi = other >> qlo;
+    uint64_t m0 = ((uint64_t)m_hi << (qlo + L)) | m_lo;
+    return {(m0 << log_n2) + i0, 1ull << (log_n2 + qlo), m_lo, m_hi};
+}
+
 // iNTT pass 1: after the DIF the value at m is the transform at bitrev(m); it is
-// stored times w_N^-(i * bitrev_L(m)) (i = i0 + ii, N = n1 * n2).
+// stored times w_N^-(i * bitrev_Ltot(m)) (i = the line, N = 2^(Ltot + log_n2)).
+// SPLIT: this launch covers index bits [qlo, qlo + L) of the Ltot-bit DIF (the
+// top bits first, in an earlier launch); only the launch with qlo = 0 applies the
+// final twiddle (final_tw).
+template <bool SPLIT = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt,
-                                                                 NttTables tb)
+                                                                 uint32_t log_n2, uint32_t lt, NttTables tb,
+                                                                 uint32_t qlo, uint32_t Ltot, uint32_t final_tw)
 {
     extern __shared__ uint64_t s[];
-    uint64_t n2 = 1ull << log_n2;
-    uint64_t i0 = (uint64_t)strided_tile(blockIdx.x, lt) << lt;
-    uint64_t *col = data + (uint64_t)blockIdx.y * cs;
+    StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
+    uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
     uint32_t tile = 1u << (L + lt);
-    strided_load(s, col, n2, i0, lt, tile);
+    strided_load(s, col, B.stride, 0, lt, tile);
     // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
     // strided tile keeps 16 values per thread whatever the plan's radix
     int q = (int)L, nr = ((int)L + 3) / 4, kr = 0;
@@ -327,62 +373,70 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
         kr++;
         q -= r;
         switch (r) {
-        case 4: strided_dif_round<16>(s, lt, q, tb); break;
-        case 3: strided_dif_round<8>(s, lt, q, tb); break;
-        case 2: strided_dif_round<4>(s, lt, q, tb); break;
-        default: strided_dif_round<2>(s, lt, q, tb); break;
+        case 4: strided_dif_round<16, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        case 3: strided_dif_round<8, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        case 2: strided_dif_round<4, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        default: strided_dif_round<2, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
         }
         __syncthreads();
     }
+    if (SPLIT && !final_tw) {
+        for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
+            col[(uint64_t)(x >> lt) * B.stride + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
+        return;
+    }
     // store: the tile holds 16 values per thread; four at a time, twiddles interleaved
-    uint32_t shift_n = 24 - (L + log_n2);
+    uint32_t Lt = SPLIT ? Ltot : L;
+    uint32_t shift_n = 2 * tb.bits - (Lt + log_n2);
+    uint64_t i0 = SPLIT ? B.base & ((1ull << log_n2) - 1) : B.base;
     for (uint32_t x0 = threadIdx.x; x0 < tile; x0 += 4 * blockDim.x) {
         uint64_t v[4], lo[4], hi[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint32_t x = x0 + k * blockDim.x;
-            uint32_t e = (uint32_t)((i0 + (x & ((1u << lt) - 1))) * bitrev32(x >> lt, L)) << shift_n;
+            uint32_t m = SPLIT ? (B.m_hi << L) | (x >> lt) : x >> lt;
+            uint32_t e = (uint32_t)((i0 + (x & ((1u << lt) - 1))) * bitrev32(m, Lt)) << shift_n;
             v[k] = s[lds_addr(x)];
-            lo[k] = tb.twl_i[e & 4095];
-            hi[k] = tb.twh_i[e >> 12];
+            lo[k] = tb.twl_i[tw_lo(e, tb)];
+            hi[k] = tb.twh_i[tw_hi(e, tb)];
         }
         gl_mul_v<4>(lo, lo, hi);
         gl_mul_v<4>(v, v, lo);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint32_t x = x0 + k * blockDim.x;
-            col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))] = v[k];
+            col[(uint64_t)(x >> lt) * B.stride + (x & ((1u << lt) - 1))] = v[k];
         }
     }
 }
 
-// NTT pass 3: n1-point DIT over m (bit-reversed in, natural out).
-template <bool INV = false>
+// NTT pass 3: n1-point DIT over m (bit-reversed in, natural out). SPLIT: this
+// launch covers index bits [qlo, qlo + L) (the low bits first, in an earlier launch).
+template <bool INV = false, bool SPLIT = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__restrict__ data, uint64_t cs, uint32_t L,
-                                                                 uint32_t log_n2, uint32_t lt,
-                                                                 NttTables tb)
+                                                                 uint32_t log_n2, uint32_t lt, NttTables tb,
+                                                                 uint32_t qlo)
 {
     extern __shared__ uint64_t s[];
-    uint64_t n2 = 1ull << log_n2;
-    uint64_t i0 = (uint64_t)strided_tile(blockIdx.x, lt) << lt;
-    uint64_t *col = data + (uint64_t)blockIdx.y * cs;
+    StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
+    uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
     uint32_t tile = 1u << (L + lt);
-    strided_load(s, col, n2, i0, lt, tile);
+    strided_load(s, col, B.stride, 0, lt, tile);
     uint32_t q = 0, nr = (L + 3) / 4, kr = 0;  // rounds as in k_ntt_strided_inv
     while (q < L) {
         uint32_t r = ((L - q) + (nr - kr) - 1) / (nr - kr);
         kr++;
         switch (r) {
-        case 4: strided_dit_round<16, INV>(s, lt, q, tb); break;
-        case 3: strided_dit_round<8, INV>(s, lt, q, tb); break;
-        case 2: strided_dit_round<4, INV>(s, lt, q, tb); break;
-        default: strided_dit_round<2, INV>(s, lt, q, tb); break;
+        case 4: strided_dit_round<16, INV, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        case 3: strided_dit_round<8, INV, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        case 2: strided_dit_round<4, INV, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
+        default: strided_dit_round<2, INV, SPLIT>(s, lt, q, tb, qlo, B.m_lo); break;
         }
         __syncthreads();
         q += r;
     }
     for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
-        col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
+        col[(uint64_t)(x >> lt) * B.stride + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
 }
 
 // ------------------------------------------------------ fused LDE block ----
@@ -416,8 +470,8 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
 #pragma unroll
             for (int j = 0; j < R; j++) {
                 uint32_t e = bitrev32((m << L2) | round_m(p, j, q, LR), log_n);
-                lo[j] = tb.shl[e & 4095];
-                hi[j] = tb.shh[e >> 12];
+                lo[j] = tb.shl[tw_lo(e, tb)];
+                hi[j] = tb.shh[tw_hi(e, tb)];
                 ni[j] = n_inv;
             }
             gl_mul_v<R>(lo, lo, hi);
@@ -457,7 +511,7 @@ __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t 
 // values. The first round reads coefficient (o_hi << r) | j for every copy
 // o_lo < B (the spread), the last one applies w_NExt^(bitrev_L1(m) * P) and
 // stores position P of block m.
-template <int R, bool INV = false>
+template <int R, bool INV = false, bool WIDE = false>
 __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64_t *__restrict__ dst, uint32_t q,
                                                 uint32_t logb, bool first, bool last, uint32_t e1, uint32_t shift_e,
                                                 uint32_t l1, const NttTables &tb)
@@ -490,16 +544,28 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
             // position P = (j << q) | o_lo of block m times w_NExt^(e1 * P) = D * C_j, with
             // D = w_NExt^(e1 * o_lo) from the split tables (once per group) and
             // C_j = w_NExt^(e1 * j * 2^q) = w_{2^(L1+LR)}^(e1 * j), one compact-table entry
-            // per j that is the same for the whole block
+            // per j that is the same for the whole block (WIDE: from the split tables,
+            // since L1 + LR may exceed the compact tables' 2^14 points)
+            const uint64_t *tl = INV ? tb.twl_i : tb.twl_f, *th = INV ? tb.twh_i : tb.twh_f;
             uint32_t ed = (e1 * p.o_lo) << shift_e;
-            uint64_t d = gl_mul((INV ? tb.twl_i : tb.twl_f)[ed & 4095], (INV ? tb.twh_i : tb.twh_f)[ed >> 12]);
+            uint64_t d = gl_mul(tl[tw_lo(ed, tb)], th[tw_hi(ed, tb)]);
             uint64_t tw[R], dd[R], c[R];
-            const uint64_t *ct = (INV ? tb.tki : tb.tkf) + (1u << (l1 + LR));
+            if (!WIDE) {
+                const uint64_t *ct = (INV ? tb.tki : tb.tkf) + (1u << (l1 + LR));
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                c[j] = ct[e1 * (uint32_t)j];
-                dd[j] = d;
+                for (int j = 1; j < R; j++) c[j] = ct[e1 * (uint32_t)j];
+            } else {
+                uint64_t cl[R], ch[R];
+#pragma unroll
+                for (int j = 1; j < R; j++) {
+                    uint32_t ec = ((e1 * (uint32_t)j) << q) << shift_e;
+                    cl[j] = tl[tw_lo(ec, tb)];
+                    ch[j] = th[tw_hi(ec, tb)];
+                }
+                gl_mul_v<R - 1>(c + 1, cl + 1, ch + 1);
             }
+#pragma unroll
+            for (int j = 1; j < R; j++) dd[j] = d;
             tw[0] = d;
             gl_mul_v<R - 1>(tw + 1, dd + 1, c + 1);
             gl_mul_v<R>(v[gg], v[gg], tw);
@@ -516,7 +582,10 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
 // (position k = coefficient bitrev_{L1+L2}(m*n2 + k), scaled); phase 1 is skipped
 // and the block only extends (computeQ's second half, and with logb = 0 the
 // block pass of a plain transform). INV: inverse roots in the DIT and twiddles.
-template <bool COEF_IN, bool INV = false>
+// WIDE: L1 > 10 (transforms above 2^24 points), where the last round's per-j
+// twiddles come from the split tables (a separate instantiation, so that the
+// common one keeps its registers).
+template <bool COEF_IN, bool INV = false, bool WIDE = false>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__restrict__ in, uint64_t cs_in,
                                                            uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
                                                            uint32_t L2, uint32_t logb, uint64_t n_inv, uint32_t rmax,
@@ -541,7 +610,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
     // phase 2: NTT DIT over bits [logb, logb + L2) of the n2*B spread values
     uint64_t *dst = out + (uint64_t)blockIdx.y * cs_out + ((uint64_t)m << (L2 + logb));
     uint32_t e1 = bitrev32(m, L1);
-    uint32_t shift_e = 24 - (log_n + logb);
+    uint32_t shift_e = 2 * tb.bits - (log_n + logb);
     uint32_t top = L2 + logb;
     uint32_t qq = logb;
     first = true;
@@ -549,10 +618,10 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
         uint32_t r = (top - qq) >= rmax ? rmax : (top - qq);
         bool last = (qq + r == top);
         switch (r) {
-        case 4: block_dit_round<16, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
-        case 3: block_dit_round<8, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
-        case 2: block_dit_round<4, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
-        default: block_dit_round<2, INV>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        case 4: block_dit_round<16, INV, WIDE>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        case 3: block_dit_round<8, INV, WIDE>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        case 2: block_dit_round<4, INV, WIDE>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+        default: block_dit_round<2, INV, WIDE>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
         }
         if (!last) __syncthreads();
         first = false;
@@ -751,19 +820,23 @@ extern "C" int ntt_init_device(void)
     if (e != hipSuccess) return (int)e;
     if (!g_attr_done) {
         int bytes = (int)(NTT_LDS_WORDS * 8);
-        e = hipFuncSetAttribute((const void *)k_ntt_strided_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_ntt_strided_fwd<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_lde_block<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_lde_block<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_lde_block<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        const void *strided[] = {(const void *)k_ntt_strided_inv<false>, (const void *)k_ntt_strided_inv<true>,
+                                 (const void *)k_ntt_strided_fwd<false, false>,
+                                 (const void *)k_ntt_strided_fwd<true, false>,
+                                 (const void *)k_ntt_strided_fwd<false, true>,
+                                 (const void *)k_ntt_strided_fwd<true, true>};
+        for (const void *f : strided)
+            if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void *)k_q_block_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        const void *blocks[] = {(const void *)k_lde_block<false, false, false>,
+                                (const void *)k_lde_block<false, false, true>,
+                                (const void *)k_lde_block<true, false, false>,
+                                (const void *)k_lde_block<true, false, true>,
+                                (const void *)k_lde_block<true, true, false>,
+                                (const void *)k_lde_block<true, true, true>};
+        for (const void *f : blocks)
+            if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return (int)e;
         g_attr_done = true;
     }
@@ -785,10 +858,70 @@ static uint32_t strided_lt(uint32_t L1, uint32_t lines_log, uint32_t tile)
     return lt > lines_log ? lines_log : lt;
 }
 
+// A strided pass over L1 index bits of m (lines 2^lines_log, stride 2^lines_log
+// per m): one launch when L1 <= NTT_MAX_STRIDED, otherwise two, each over about
+// half the bits (the DIF of the inverse pass takes the top bits first, the DIT of
+// the forward pass the low bits first). Part k covers bits [qlo, qlo + L) with
+// runs of 2^lt words.
+struct StridedPart {
+    uint32_t L, qlo, lt;
+};
+static int strided_parts(uint32_t L1, uint32_t lines_log, uint32_t tile, bool top_first, StridedPart *part)
+{
+    if (L1 == 0) return 0;
+    if (L1 <= NTT_MAX_STRIDED) {
+        part[0] = {L1, 0, strided_lt(L1, lines_log, tile)};
+        return 1;
+    }
+    uint32_t lo = L1 / 2, hi = L1 - lo;  // bits [0, lo) and [lo, L1)
+    StridedPart a = {hi, lo, strided_lt(hi, lines_log, tile)}, b = {lo, 0, strided_lt(lo, lines_log, tile)};
+    part[0] = top_first ? a : b;
+    part[1] = top_first ? b : a;
+    return 2;
+}
+
+static void launch_strided_inv(uint64_t *data, uint64_t cs, uint32_t L1, uint32_t log_n2, uint32_t tile,
+                               uint64_t ncols, const NttTables &tb, hipStream_t st)
+{
+    StridedPart part[2];
+    int np = strided_parts(L1, log_n2, tile, true, part);
+    for (int k = 0; k < np; k++) {
+        const StridedPart &P = part[k];
+        uint32_t log_tile = P.L + P.lt;
+        dim3 g((unsigned)((1ull << (log_n2 - P.lt)) << (L1 - P.L)), (unsigned)ncols);
+        if (np == 1)
+            hipLaunchKernelGGL(k_ntt_strided_inv<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, data,
+                               cs, P.L, log_n2, P.lt, tb, 0u, L1, 1u);
+        else
+            hipLaunchKernelGGL(k_ntt_strided_inv<true>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, data,
+                               cs, P.L, log_n2, P.lt, tb, P.qlo, L1, (uint32_t)(k == np - 1));
+    }
+}
+
+template <bool INV>
+static void launch_strided_fwd(uint64_t *data, uint64_t cs, uint32_t L1, uint32_t log_n2, uint32_t tile,
+                               uint64_t ncols, const NttTables &tb, hipStream_t st)
+{
+    StridedPart part[2];
+    int np = strided_parts(L1, log_n2, tile, false, part);
+    for (int k = 0; k < np; k++) {
+        const StridedPart &P = part[k];
+        uint32_t log_tile = P.L + P.lt;
+        dim3 g((unsigned)((1ull << (log_n2 - P.lt)) << (L1 - P.L)), (unsigned)ncols);
+        if (np == 1)
+            hipLaunchKernelGGL((k_ntt_strided_fwd<INV, false>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st,
+                               data, cs, P.L, log_n2, P.lt, tb, 0u);
+        else
+            hipLaunchKernelGGL((k_ntt_strided_fwd<INV, true>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st,
+                               data, cs, P.L, log_n2, P.lt, tb, P.qlo);
+    }
+}
+
 // Launch geometry of an LDE of 2^log_n -> 2^log_ext rows under a plan: n2 * B <=
-// 2^tile values per block (tile <= 14), n1 = N / n2 <= 2^10; a shape that needs
-// n1 > 2^10 at the requested tile gets the smallest tile that fits. Returns
-// nonzero for a shape no plan covers (log_n >= 4 assumed).
+// 2^tile values per block (tile <= 14), n1 = N / n2 <= 2^10 for one strided launch;
+// a shape that needs n1 > 2^10 at the requested tile gets the smallest tile that
+// fits, and beyond 2^14 values per block the strided passes take two launches
+// (n1 <= 2^20). Returns nonzero for a shape no plan covers (log_n >= 4 assumed).
 struct LdeGeometry {
     uint32_t tile, rmax, L1, L2, lt_inv, lt_fwd;
 };
@@ -801,10 +934,13 @@ static int lde_geometry(uint32_t log_n, uint32_t log_ext, const NttPlan *plan, L
     if (rmax < 1 || rmax > 4) rmax = 4;
     uint32_t L2 = log_n;
     if (L2 + log_b > tile) L2 = tile - log_b;
-    if (log_n - L2 > 10) L2 = log_n - 10;
+    if (log_n - L2 > NTT_MAX_STRIDED) L2 = std::min(log_n - NTT_MAX_STRIDED, (uint32_t)NTT_LOG_TILE - log_b);
     uint32_t L1 = log_n - L2;
-    if (L2 + log_b > NTT_LOG_TILE || L2 < 4) return -2;  // n2*B <= 2^14 and >= 16 values per block line
-    *g = {tile, rmax, L1, L2, L1 ? strided_lt(L1, L2, tile) : 0, L1 ? strided_lt(L1, L2 + log_b, tile) : 0};
+    if (L2 + log_b > NTT_LOG_TILE || L2 < 4 || L1 > 2 * NTT_MAX_STRIDED) return -2;  // n2*B <= 2^14, >= 16 values
+    StridedPart pi[2], pf[2];
+    strided_parts(L1, L2, tile, true, pi);
+    strided_parts(L1, L2 + log_b, tile, false, pf);
+    *g = {tile, rmax, L1, L2, L1 ? pi[0].lt : 0, L1 ? pf[0].lt : 0};
     return 0;
 }
 
@@ -821,7 +957,7 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
                               hipEvent_t *ev)
 {
     // ev (optional): events recorded after passes 0..3 (ev[0..3]) for phase timing
-    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = tables_for(th, log_ext);
     uint64_t N = 1ull << log_n;
     if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
     uint32_t log_b = log_ext - log_n;
@@ -846,31 +982,22 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
     uint32_t rmax = geo.rmax, L1 = geo.L1, L2 = geo.L2;
 
     // 1. strided inverse DIF over n1
-    if (L1 > 0) {
-        uint32_t lt = geo.lt_inv;
-        uint32_t log_tile = L1 + lt;
-        dim3 g((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
-        hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, L1,
-                           L2, lt, tb);
-    }
+    launch_strided_inv(scratch, N, L1, L2, geo.tile, ncols, tb, st);
     if (ev) (void)hipEventRecord(ev[1], st);
     // 2. fused block: DIF, coset scale, spread, DIT, twiddle
     {
         uint32_t log_tile = L2 + log_b;
         dim3 g((unsigned)(1u << L1), (unsigned)ncols);
-        hipLaunchKernelGGL(k_lde_block<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
-                           N << log_b, L1, L2, log_b, n_inv, rmax, tb);
+        if (L1 > NTT_MAX_STRIDED)
+            hipLaunchKernelGGL((k_lde_block<false, false, true>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile),
+                               st, scratch, N, lde, N << log_b, L1, L2, log_b, n_inv, rmax, tb);
+        else
+            hipLaunchKernelGGL((k_lde_block<false, false, false>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile),
+                               st, scratch, N, lde, N << log_b, L1, L2, log_b, n_inv, rmax, tb);
     }
     if (ev) (void)hipEventRecord(ev[2], st);
     // 3. strided forward DIT over n1
-    if (L1 > 0) {
-        uint32_t l2b = L2 + log_b;
-        uint32_t lt = geo.lt_fwd;
-        uint32_t log_tile = L1 + lt;
-        dim3 g((unsigned)(1ull << (l2b - lt)), (unsigned)ncols);
-        hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, lde,
-                           N << log_b, L1, l2b, lt, tb);
-    }
+    launch_strided_fwd<false>(lde, N << log_b, L1, L2 + log_b, geo.tile, ncols, tb, st);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }
@@ -879,7 +1006,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
                                     uint32_t log_n, uint32_t log_ext, uint32_t q_deg, uint32_t q_dim,
                                     const NttTablesHost *th, hipStream_t st)
 {
-    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = tables_for(th, log_ext);
     if (log_ext < log_n || log_ext > NTT_MAX_LOG) return -3;
     uint32_t log_b = log_ext - log_n;
     if (q_deg == 0 || q_dim == 0 || q_deg > (1u << log_b)) return -3;
@@ -898,7 +1025,7 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
     uint32_t L2 = log_n;
     if (L2 + log_b > NTT_LOG_TILE) L2 = NTT_LOG_TILE - log_b;
     uint32_t L1 = log_n - L2, lb = L2 + log_b;
-    if (L1 > 10 || L2 < 1 || lb < 4) return -2;
+    if (L1 > 2 * NTT_MAX_STRIDED || L2 < 1 || lb < 4) return -2;
 
     // 0. the q columns, column-major, into the last qDim columns of cmQ (consumed
     //    before the extension writes there)
@@ -911,27 +1038,19 @@ extern "C" int ntt_launch_compute_q(uint64_t *cmq, const uint64_t *q, uint64_t q
         hipLaunchKernelGGL(k_transpose_rm2cm, tg, dim3(256), 0, st, q, qc, NExt, (uint64_t)q_dim, q_rs, NExt);
     }
     // 1. strided inverse DIF over n1 (stride n2*B)
-    if (L1 > 0) {
-        uint32_t lt = strided_lt(L1, lb, NTT_STRIDED_TILE);
-        uint32_t log_tile = L1 + lt;
-        dim3 g((unsigned)(1ull << (lb - lt)), q_dim);
-        hipLaunchKernelGGL(k_ntt_strided_inv, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, qc, NExt, L1,
-                           lb, lt, tb);
-    }
+    launch_strided_inv(qc, NExt, L1, lb, NTT_STRIDED_TILE, q_dim, tb, st);
     // 2a. inverse blocks -> scaled coefficient lines (coef: column c at coef + c*N)
     hipLaunchKernelGGL(k_q_block_inv, dim3(1u << L1, q_dim), dim3(1u << (lb - 4)), lds_bytes(lb), st, qc, NExt, coef,
                        N, L1, L2, log_b, q_deg, q_dim, n_inv_ext, shift_in, tb);
     // 2b. extension blocks (spread, DIT, twiddle) into cmQ
-    hipLaunchKernelGGL(k_lde_block<true>, dim3(1u << L1, ncols), dim3(1u << (lb - 4)), lds_bytes(lb), st, coef, N,
-                       cmq, NExt, L1, L2, log_b, (uint64_t)0, 4u, tb);
+    if (L1 > NTT_MAX_STRIDED)
+        hipLaunchKernelGGL((k_lde_block<true, false, true>), dim3(1u << L1, ncols), dim3(1u << (lb - 4)), lds_bytes(lb),
+                           st, coef, N, cmq, NExt, L1, L2, log_b, (uint64_t)0, 4u, tb);
+    else
+        hipLaunchKernelGGL((k_lde_block<true, false, false>), dim3(1u << L1, ncols), dim3(1u << (lb - 4)),
+                           lds_bytes(lb), st, coef, N, cmq, NExt, L1, L2, log_b, (uint64_t)0, 4u, tb);
     // 3. strided forward DIT over n1
-    if (L1 > 0) {
-        uint32_t lt = strided_lt(L1, lb, NTT_STRIDED_TILE);
-        uint32_t log_tile = L1 + lt;
-        dim3 g((unsigned)(1ull << (lb - lt)), ncols);
-        hipLaunchKernelGGL(k_ntt_strided_fwd<false>, g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, cmq, NExt, L1,
-                           lb, lt, tb);
-    }
+    launch_strided_fwd<false>(cmq, NExt, L1, lb, NTT_STRIDED_TILE, ncols, tb, st);
     return (int)hipGetLastError();
 }
 
@@ -951,7 +1070,7 @@ extern "C" int ntt_launch_ntt_tiny(uint64_t *dst, const uint64_t *src, uint64_t 
 extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, uint64_t ncols, int inverse,
                               const NttTablesHost *th, hipStream_t st)
 {
-    NttTables tb = {th->tkf, th->tki, th->twl_f, th->twh_f, th->twl_i, th->twh_i, th->shl, th->shh};
+    NttTables tb = tables_for(th, lm);
     if (lm > NTT_MAX_LOG) return -3;
     uint64_t M = 1ull << lm;
     uint64_t scale = inverse ? gl_inv(M % GL_P) : 1;
@@ -964,25 +1083,18 @@ extern "C" int ntt_launch_ntt(uint64_t *dst, const uint64_t *src, uint32_t lm, u
         hipLaunchKernelGGL(k_bitrev_small, dim3((unsigned)ncols), dim3(256), 0, st, src, dst, M, lm, scale);
     // 2. block DIT over the low L2 bits, then 3. strided DIT over n1
     uint32_t L2 = lm < NTT_LOG_TILE ? lm : NTT_LOG_TILE, L1 = lm - L2;
-    if (L1 > 10) return -2;
+    if (L1 > 2 * NTT_MAX_STRIDED) return -2;
     dim3 g(1u << L1, (unsigned)ncols);
     if (inverse)
-        hipLaunchKernelGGL((k_lde_block<true, true>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
-                           L2, 0u, (uint64_t)0, 4u, tb);
+        hipLaunchKernelGGL((L1 > NTT_MAX_STRIDED ? k_lde_block<true, true, true> : k_lde_block<true, true, false>), g,
+                           dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1, L2, 0u, (uint64_t)0, 4u, tb);
     else
-        hipLaunchKernelGGL((k_lde_block<true, false>), g, dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1,
-                           L2, 0u, (uint64_t)0, 4u, tb);
-    if (L1 > 0) {
-        uint32_t lt = strided_lt(L1, L2, NTT_STRIDED_TILE);
-        uint32_t log_tile = L1 + lt;
-        dim3 gs((unsigned)(1ull << (L2 - lt)), (unsigned)ncols);
-        if (inverse)
-            hipLaunchKernelGGL(k_ntt_strided_fwd<true>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst, M,
-                               L1, L2, lt, tb);
-        else
-            hipLaunchKernelGGL(k_ntt_strided_fwd<false>, gs, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, dst,
-                               M, L1, L2, lt, tb);
-    }
+        hipLaunchKernelGGL((L1 > NTT_MAX_STRIDED ? k_lde_block<true, false, true> : k_lde_block<true, false, false>), g,
+                           dim3(1u << (L2 - 4)), lds_bytes(L2), st, dst, M, dst, M, L1, L2, 0u, (uint64_t)0, 4u, tb);
+    if (inverse)
+        launch_strided_fwd<true>(dst, M, L1, L2, NTT_STRIDED_TILE, ncols, tb, st);
+    else
+        launch_strided_fwd<false>(dst, M, L1, L2, NTT_STRIDED_TILE, ncols, tb, st);
     return (int)hipGetLastError();
 }
 

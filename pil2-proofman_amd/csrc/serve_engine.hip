@@ -365,7 +365,7 @@ int64_t check_request(const Server *s, const Pil2ServeRequest *r)
 {
     if (!r) return api_fail(-13, "null request");
     if (r->n_bits_ext < r->n_bits) return api_fail(-10, "nBitsExt < nBits");
-    if (r->n_bits_ext > NTT_MAX_LOG) return api_fail(-11, "nBitsExt > 24 is not supported");
+    if (r->n_bits_ext > NTT_MAX_LOG) return api_fail(-11, "nBitsExt > 28 is not supported");
     if (r->n_cols > 65535) return api_fail(-12, "nCols > 65535 is not supported");
     if (r->n_queries && !r->h_queries) return api_fail(-13, "null queries");
     uint64_t ne = 1ull << r->n_bits_ext;

@@ -59,11 +59,31 @@ static const uint64_t kRoots[33] = {
 // that no entry point allocates for them: each device holds its own copy, found by
 // hipGetSymbolAddress with that device current.
 static const int kMaxDevices = 16;
-static const size_t kTabRoots = (size_t)2 << NTT_LOG_TILE, kTabTw = 4096;
-static const size_t kTabWords = 2 * kTabRoots + 6 * kTabTw;
+// compact in-tile tables (2 x 2^15 words), then the split tables of the 2^24 domain
+// (6 x 2^12 words) and of the 2^28 domain (6 x 2^14 words)
+static const size_t kTabRoots = (size_t)2 << NTT_LOG_TILE, kTab24 = (size_t)1 << 12, kTab28 = (size_t)1 << 14;
+static const size_t kTabWords = 2 * kTabRoots + 6 * kTab24 + 6 * kTab28;
 static __device__ __align__(256) uint64_t g_tab_dev[kTabWords];
 static NttTablesHost g_tables[kMaxDevices];
 static bool g_tables_ready[kMaxDevices];
+
+// Split tables of the 2^(2*bits) domain at t: w^x = lo[x & (2^bits - 1)] * hi[x >> bits]
+// for w = W[2*bits] (forward and inverse), and 7^e the same way.
+static void split_tables(uint64_t *t, uint32_t bits)
+{
+    const size_t nt = (size_t)1 << bits;
+    uint64_t w = kRoots[2 * bits], wi = gl_inv(w);
+    uint64_t *twl_f = t, *twh_f = t + nt, *twl_i = t + 2 * nt, *twh_i = t + 3 * nt, *shl = t + 4 * nt,
+             *shh = t + 5 * nt;
+    uint64_t wn = gl_pow(w, nt), win = gl_pow(wi, nt), sn = gl_pow(7, nt);
+    uint64_t a = 1, b = 1, c = 1, d = 1, e = 1, f = 1;
+    for (size_t j = 0; j < nt; j++) {
+        twl_f[j] = a; twh_f[j] = b; twl_i[j] = c; twh_i[j] = d; shl[j] = e; shh[j] = f;
+        a = gl_mul(a, w); b = gl_mul(b, wn);
+        c = gl_mul(c, wi); d = gl_mul(d, win);
+        e = gl_mul(e, 7); f = gl_mul(f, sn);
+    }
+}
 
 static int64_t ensure_tables(NttTablesHost **out)
 {
@@ -72,9 +92,9 @@ static int64_t ensure_tables(NttTablesHost **out)
     if (dev < 0 || dev >= kMaxDevices) return fail(-4, "device index out of range");
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_tables_ready[dev]) {
-        const size_t nr = kTabRoots, nt = kTabTw;
+        const size_t nr = kTabRoots;
         std::vector<uint64_t> h(kTabWords);
-        uint64_t *rf = h.data(), *ri = rf + nr, *t = ri + nr;
+        uint64_t *rf = h.data(), *ri = rf + nr, *t24 = ri + nr, *t28 = t24 + 6 * kTab24;
         for (uint32_t k = 0; k <= NTT_LOG_TILE; k++) {  // rf[2^k + y] = w_{2^k}^y, ri[2^k + y] = w_{2^k}^-y
             uint64_t w = kRoots[k], wi = gl_inv(w), a = 1, b = 1;
             for (size_t y = 0; y < ((size_t)1 << k); y++) {
@@ -87,26 +107,22 @@ static int64_t ensure_tables(NttTablesHost **out)
         rf[0] = ri[0] = 1;  // unused slot
         int ie = ntt_init_device();
         if (ie) return fail(-100 - ie, "NTT device init failed");
-        uint64_t w24 = kRoots[24], w24i = gl_inv(w24);
-        uint64_t *twl_f = t, *twh_f = t + nt, *twl_i = t + 2 * nt, *twh_i = t + 3 * nt, *shl = t + 4 * nt,
-                 *shh = t + 5 * nt;
-        uint64_t w24_4096 = gl_pow(w24, 4096), w24i_4096 = gl_pow(w24i, 4096), s4096 = gl_pow(7, 4096);
-        uint64_t a = 1, b = 1, c = 1, d = 1, e = 1, f = 1;
-        for (size_t j = 0; j < nt; j++) {
-            twl_f[j] = a; twh_f[j] = b; twl_i[j] = c; twh_i[j] = d; shl[j] = e; shh[j] = f;
-            a = gl_mul(a, w24); b = gl_mul(b, w24_4096);
-            c = gl_mul(c, w24i); d = gl_mul(d, w24i_4096);
-            e = gl_mul(e, 7); f = gl_mul(f, s4096);
-        }
+        split_tables(t24, 12);
+        split_tables(t28, 14);
         uint64_t *dptr = nullptr;
         HIP_TRY(hipGetSymbolAddress((void **)&dptr, HIP_SYMBOL(g_tab_dev)), "hipGetSymbolAddress(tables)");
         HIP_TRY(hipMemcpy(dptr, h.data(), h.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(tables)");
         NttTablesHost &T = g_tables[dev];
         T.tkf = dptr;
         T.tki = dptr + nr;
-        uint64_t *dt = dptr + 2 * nr;
-        T.twl_f = dt; T.twh_f = dt + nt; T.twl_i = dt + 2 * nt; T.twh_i = dt + 3 * nt; T.shl = dt + 4 * nt;
-        T.shh = dt + 5 * nt;
+        for (int d = 0; d < 2; d++) {
+            size_t nt = d ? kTab28 : kTab24;
+            uint64_t *dt = dptr + 2 * nr + (d ? 6 * kTab24 : 0);
+            NttSplitTables &S = T.split[d];
+            S.twl_f = dt; S.twh_f = dt + nt; S.twl_i = dt + 2 * nt; S.twh_i = dt + 3 * nt; S.shl = dt + 4 * nt;
+            S.shh = dt + 5 * nt;
+            S.bits = d ? 14 : 12;
+        }
         g_tables_ready[dev] = true;
     }
     *out = &g_tables[dev];
@@ -159,7 +175,7 @@ static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_sc
                         const Pil2CommitPlan &plan = kDefaultPlan)
 {
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (nCols == 0) return 0;
     if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
     if (!d_lde || !d_trace || !d_scratch) return fail(-13, "null buffer");
@@ -182,7 +198,7 @@ static int64_t commit_impl(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *d
     int64_t rc = check_family();
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (!d_nodes) return fail(-13, "null buffer");
     if (ev) (void)hipEventRecord(g_ev[0], st);
     if ((rc = lde_impl(d_lde, d_trace, d_scratch, nBits, nBitsExt, nCols, st, ev, plan))) return rc;
@@ -386,7 +402,7 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
     if (!d_nodes || (nCols && (!d_lde || !h_trace || !d_stage))) return fail(-13, "null buffer");
     uint64_t rate = 4 * arity - 4, w = stream_group_cols(nBits, nCols, arity, groupCols);
@@ -458,7 +474,7 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
 int64_t pil2hip_ntt(uint64_t *d_dst, const uint64_t *d_src, uint64_t *d_scratch, uint64_t nBits, uint64_t nCols,
                     bool inverse, bool rowMajor, void *stream)
 {
-    if (nBits > NTT_MAX_LOG) return fail(-11, "size > 2^24 is not supported");
+    if (nBits > NTT_MAX_LOG) return fail(-11, "size > 2^28 is not supported");
     if (nCols == 0) return 0;
     if (nCols > 65535) return fail(-12, "nCols > 65535 is not supported");
     if (!d_src) return fail(-13, "null buffer");
@@ -502,7 +518,7 @@ int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_
         if ((rc = check_arity(arity))) return rc;
     }
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (qDeg == 0 || qDim == 0) return fail(-12, "qDeg and qDim must be >= 1");
     if (qDeg > (1ull << (nBitsExt - nBits))) return fail(-12, "qDeg exceeds the blowup factor NExt / N");
     if (qDeg * qDim > 65535) return fail(-12, "qDeg * qDim > 65535 is not supported");
@@ -1392,7 +1408,7 @@ static int64_t host_commit(void *root, uint64_t arity, uint64_t nBits, uint64_t 
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (!root || (!buffer && nCols)) return fail(-13, "null buffer");
     uint64_t N = 1ull << nBits, NExt = 1ull << nBitsExt;
     uint64_t nTree = pil2hip_tree_num_elements(NExt, arity);
@@ -1488,7 +1504,7 @@ int64_t pil2hip_const_tree(uint64_t *d_tree, const uint64_t *d_constPols, uint64
     if (rc) return rc;
     if ((rc = check_arity(arity))) return rc;
     if (nBitsExt < nBits) return fail(-10, "nBitsExt < nBits");
-    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 24 is not supported");
+    if (nBitsExt > NTT_MAX_LOG) return fail(-11, "nBitsExt > 28 is not supported");
     if (!d_tree || (nConstants && (!d_constPols || !d_scratch))) return fail(-13, "null buffer");
     uint64_t NExt = 1ull << nBitsExt, C = nConstants;
     hipStream_t st = (hipStream_t)stream;

@@ -18,7 +18,13 @@ compiled_reference.json
     NTT around the regrouping of starks.hpp:209-221). Digest = sha256 of the
     canonical row-major array (little endian).
 
-Usage: python tests/golden/make_fixtures.py
+config_reference.json   (--configs; about 40 min on 8 cores, 40 GB of host memory)
+    The configurations at their real sizes, from the same compiled reference: the
+    plain-large root, LDE digest and tree digest (2^22 x 128 -> 2^24, arity 3), the
+    roots of the 64-instance mixed batch, and root + group-proof digest for the
+    first 64 jobs of the serving stream (log2 rows 12..22).
+
+Usage: python tests/golden/make_fixtures.py [--configs]
 """
 import hashlib
 import json
@@ -36,6 +42,7 @@ from oracle.oracle import RefLib, build  # noqa: E402
 import datagen  # noqa: E402
 
 REF_TESTS = "/root/reference/pil2-stark/src/goldilocks/tests"
+P = 0xFFFFFFFF00000001
 
 
 def _hexes(text):
@@ -138,8 +145,165 @@ def compute_q_cases(ref):
     return out
 
 
+def _ref_commit(ref, orc, n_bits, n_bits_ext, n_cols, seed, arity, keep_lde=False):
+    """The reference's CPU commit of one seeded trace at its full size, lean in host
+    memory (the largest serving job's LDE alone is 29 GiB): trace from the C
+    generator, NTT_Goldilocks::LDE straight into a buffer with the 8 rows of slack
+    the reference merkletree touches, Poseidon2Goldilocks<W>::merkletree on it.
+    Returns (lde buffer of NExt + 8 rows, tree), the trace already freed."""
+    n, ne = 1 << n_bits, 1 << n_bits_ext
+    trace = np.empty((n, n_cols), dtype=np.uint64)
+    orc.L.oracle_fill_uniform(_p(trace), trace.size, seed)
+    lde = np.zeros((ne + 8, n_cols), dtype=np.uint64)
+    if 8 * ne * n_cols <= (4 << 30):
+        ref.L.ref_lde(_p(lde), _p(trace), ne, n, n_cols)
+    else:
+        # NTT_Goldilocks::LDE allocates a temporary as large as its output: extend
+        # groups of columns (each column is independent) and place them
+        g = max(1, (256 << 20) // ne)
+        for c0 in range(0, n_cols, g):
+            c1 = min(n_cols, c0 + g)
+            part_in = np.ascontiguousarray(trace[:, c0:c1])
+            part_out = np.empty((ne, c1 - c0), dtype=np.uint64)
+            ref.L.ref_lde(_p(part_out), _p(part_in), ne, n, c1 - c0)
+            lde[:ne, c0:c1] = part_out
+            del part_in, part_out
+    del trace
+    tl = orc.tree_num_elements(ne, arity)
+    tree = np.zeros(tl + 8 * 4 * 16, dtype=np.uint64)
+    assert ref.L.ref_merkletree(_p(tree), _p(lde), n_cols, ne, arity) == 0
+    tree = tree[:tl]
+    np.remainder(tree, np.uint64(P), out=tree)
+    return lde, tree
+
+
+def _p(a):
+    import ctypes
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _canonical_sha256(lde, rows):
+    """sha256 of the first `rows` rows, canonical, little endian, in row blocks."""
+    h = hashlib.sha256()
+    step = max(1, (1 << 27) // max(1, lde.shape[1]))
+    for r in range(0, rows, step):
+        blk = np.remainder(lde[r:min(rows, r + step)], np.uint64(P))
+        h.update(np.ascontiguousarray(blk, dtype="<u8").tobytes())
+    return h.hexdigest()
+
+
+SERVE_QUERY_SALT = 0xA5A5A5A5
+
+
+def config_reference(ref, n_serve=64):
+    """Bit-exact expectations of the configurations at their real sizes, from the
+    reference's own CPU code (oracle/_ref): the plain-large root and LDE digest, the
+    64 batched roots, and for a prefix of the serving stream each job's root and the
+    sha256 of its 32 group proofs (row values + sibling path to the last verified
+    level, merkleTreeGL.cpp:145-175, taken by the oracle's opening from the
+    reference's LDE and tree)."""
+    from oracle.oracle import Oracle
+    import time
+    orc = Oracle()
+    out = {"generator": "datagen.uniform(rows*cols, seed) row-major", "library": os.path.basename(ref.path)}
+    t0 = time.time()
+    # plain large: 2^22 x 128, blowup 4, arity 3
+    lde, tree = _ref_commit(ref, orc, 22, 24, 128, 0x5EED0002, 3)
+    out["plain_large"] = {"n_bits": 22, "n_bits_ext": 24, "n_cols": 128, "seed": 0x5EED0002, "arity": 3,
+                          "root": ["0x%016x" % v for v in tree[-4:]],
+                          "lde_sha256": _canonical_sha256(lde, 1 << 24),
+                          "tree_sha256": hashlib.sha256(tree.astype("<u8").tobytes()).hexdigest()}
+    del lde, tree
+    print("plain_large done %.0f s" % (time.time() - t0), flush=True)
+    # batched: 64 instances, seed 0x5EED0003, arity 3
+    shapes = datagen.batched_workload(seed=0x5EED0003, n_instances=64)
+    roots = []
+    for k, j in enumerate(shapes):
+        lde, tree = _ref_commit(ref, orc, j["n_bits"], j["n_bits_ext"], j["n_cols"], j["seed"], 3)
+        roots.append(["0x%016x" % v for v in tree[-4:]])
+        del lde, tree
+        print("batched %d/64 done %.0f s" % (k + 1, time.time() - t0), flush=True)
+    out["batched"] = {"seed": 0x5EED0003, "n_instances": 64, "arity": 3, "shapes": shapes, "roots": roots}
+    # serving: the first n_serve jobs of the stream (seed 0x5EED0005, log2 rows 12..22),
+    # 32 queries each drawn as in the serving benchmark, llv 2, arity 3
+    nq, llv, arity = 32, 2, 3
+    shapes = datagen.batched_workload(seed=0x5EED0005, n_instances=n_serve, min_log=12, max_log=22)
+    qwords = datagen.raw_words(n_serve * nq, 0x5EED0005 ^ SERVE_QUERY_SALT).reshape(n_serve, nq)
+    jobs = []
+    for k, j in enumerate(shapes):
+        ne = 1 << j["n_bits_ext"]
+        lde, tree = _ref_commit(ref, orc, j["n_bits"], j["n_bits_ext"], j["n_cols"], j["seed"], arity)
+        q = [int(v) % ne for v in qwords[k]]
+        h = hashlib.sha256()
+        for idx in q:
+            buf = np.zeros(j["n_cols"] + 64 * 4 * arity, dtype=np.uint64)
+            nw = orc.L.oracle_merkle_open(_p(buf), _p(tree), _p(lde), j["n_cols"], ne, j["n_cols"], 1, arity, llv,
+                                          idx)
+            h.update(buf[:nw].astype("<u8").tobytes())
+        jobs.append(dict(j, root=["0x%016x" % v for v in tree[-4:]], queries=q, proofs_sha256=h.hexdigest()))
+        del lde, tree
+        print("serving %d/%d done %.0f s" % (k + 1, n_serve, time.time() - t0), flush=True)
+    out["serving"] = {"seed": 0x5EED0005, "query_seed": 0x5EED0005 ^ SERVE_QUERY_SALT, "queries_per_job": nq,
+                      "llv": llv, "arity": arity, "jobs": jobs}
+    return out
+
+
+def large_domain_reference(ref):
+    """Transforms above 2^24 points (the strided passes then split into two
+    launches and the twiddles come from the 2^28-domain tables): LDE digests and
+    arity-3 roots of NTT_Goldilocks::LDE + Poseidon2Goldilocks<12>::merkletree, plain
+    NTT / INTT digests, and one computeQ digest, all from the compiled reference."""
+    from oracle.oracle import Oracle
+    orc = Oracle()
+    out = {"generator": "datagen.uniform(rows*cols, seed) row-major", "library": os.path.basename(ref.path),
+           "lde": [], "ntt": [], "compute_q": []}
+    for nb, nbe, c, seed in [(24, 25, 2, 0xD0025), (23, 25, 3, 0xD1025), (27, 28, 1, 0xD0028), (26, 28, 1, 0xD1028),
+                             (21, 25, 4, 0xD2025)]:
+        lde, tree = _ref_commit(ref, orc, nb, nbe, c, seed, 3)
+        out["lde"].append({"n_bits": nb, "n_bits_ext": nbe, "n_cols": c, "seed": seed, "arity": 3,
+                           "lde_sha256": _canonical_sha256(lde, 1 << nbe),
+                           "root": ["0x%016x" % v for v in tree[-4:]]})
+        del lde, tree
+        print("large LDE", nb, nbe, c, "done", flush=True)
+    for lg, c, seed in [(25, 1, 0xE025), (28, 1, 0xE028)]:
+        a = np.empty((1 << lg, c), dtype=np.uint64)
+        orc.L.oracle_fill_uniform(_p(a), a.size, seed)
+        f = a.copy()
+        ref.L.ref_ntt(_p(f), 1 << lg, c, 0)
+        d_ntt = _canonical_sha256(f, 1 << lg)
+        del f
+        ref.L.ref_ntt(_p(a), 1 << lg, c, 1)
+        out["ntt"].append({"log_size": lg, "n_cols": c, "seed": seed, "ntt_sha256": d_ntt,
+                           "intt_sha256": _canonical_sha256(a, 1 << lg)})
+        del a
+        print("large NTT", lg, "done", flush=True)
+    for lb, lbe, qdeg, qdim, seed in [(23, 25, 2, 1, 0xF025)]:
+        ne = 1 << lbe
+        q = np.empty((ne, qdim), dtype=np.uint64)
+        orc.L.oracle_fill_uniform(_p(q), q.size, seed)
+        cmq = np.zeros((ne, qdeg * qdim), dtype=np.uint64)
+        ref.L.ref_compute_q(_p(cmq), _p(q), 1 << lb, ne, qdeg, qdim)
+        out["compute_q"].append({"n_bits": lb, "n_bits_ext": lbe, "q_deg": qdeg, "q_dim": qdim, "seed": seed,
+                                 "cmq_sha256": _canonical_sha256(cmq, ne)})
+        print("large computeQ done", flush=True)
+    return out
+
+
 def main():
     build()
+    if "--large" in sys.argv:
+        ref = RefLib()
+        with open(os.path.join(HERE, "large_domain_reference.json"), "w") as f:
+            json.dump(large_domain_reference(ref), f, indent=1)
+        print("large-domain fixtures written")
+        return
+    if "--configs" in sys.argv:
+        ref = RefLib()
+        with open(os.path.join(HERE, "config_reference.json"), "w") as f:
+            json.dump(config_reference(ref), f, indent=1)
+        print("config fixtures written")
+        return
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(reference_vectors(), f, indent=1)
     ref = RefLib()
