@@ -158,6 +158,26 @@ def cpu_baseline_batched(shapes, arity, target_s=15.0, what="batch"):
             **host_info()}
 
 
+_MARKER = []
+
+
+def profile_region(args, on):
+    """With --profile-region, mark the start (on) and end of a timed region in a
+    profile: a one-block k_fill_uniform launch (grid 256, unlike any input fill),
+    with the device idle on both sides. The profile summaries keep the dispatches
+    between the two markers: the timed steps only, without warm-up, tuning,
+    calibration or the per-instance pass."""
+    if not getattr(args, "profile_region", False):
+        return
+    import torch
+    import pil2_commit as pc
+    if not _MARKER:
+        _MARKER.append(pc.empty(1))
+    torch.cuda.synchronize()
+    pc.fill_uniform(_MARKER[0], 1 if on else 2)
+    torch.cuda.synchronize()
+
+
 _MUL_PEAK = []
 
 
@@ -347,11 +367,13 @@ def run_batched(args, cfg, embedded=False):
     for _ in range(b_warm):
         batch.launch(plan=request)
     torch.cuda.synchronize()
+    profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(b_steps):
         batch.launch(plan=request)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    profile_region(args, False)
     value = elems / (dt / b_steps) / 1e9
     roots = batch.roots_host()
     match = all(list(map(int, roots[k])) == list(map(int, plain_roots[k])) for k in range(len(shapes)))
@@ -471,12 +493,13 @@ def run_fri(args, cfg):
     d_ch = pc.to_device(np.asarray(chal, dtype=np.uint64).reshape(-1), dev)
     for _ in range(args.warmup):
         pc.FRI.layers(src, work, d_ch, nbe, steps, trees, arity=arity, graph=True)
-    torch.cuda.synchronize()
+    profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pc.FRI.layers(src, work, d_ch, nbe, steps, trees, arity=arity, graph=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    profile_region(args, False)
     graph_roots = [["%016x" % v for v in t.get_root()] for t in trees]
     graph_final = pc.to_host(work)[:3 << steps[-1]]
     # the same loop launched call by call (per-layer entry points), for comparison
@@ -729,12 +752,13 @@ def run_compute_q(args, cfg):
     step = lambda: pc.compute_q(q, nb, nbe, qdeg, q_dim=qdim, arity=arity, cmq=cmq, nodes=nodes, scratch=scratch)
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    profile_region(args, False)
     # phases on the same stream: the transform alone, then the tree alone
     st = torch.cuda.current_stream()
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -943,6 +967,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-h2d", action="store_true",
                     help="plain configs: skip the passes from host memory (the with_h2d object)")
+    ap.add_argument("--profile-region", action="store_true",
+                    help="mark the timed steps in a profile with one-block fill launches before and after")
     ap.add_argument("--plain-only", action="store_true",
                     help="plain_large: leave out the embedded batched and serving runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -1028,12 +1054,13 @@ def run_plain(args, cfg):
     # timed region: K commits; phase events on the commit stream
     pc.lib.pil2hip_timing_read(None, True)
     pc.lib.pil2hip_timing_enable(True)
-    torch.cuda.synchronize()
+    profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pc.commit(trace, arity, nb, nbe, cols, lde=lde, nodes=nodes, scratch=scratch)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    profile_region(args, False)
     pc.lib.pil2hip_timing_enable(False)
     ph = (ctypes.c_double * 6)()
     ncalls = pc.lib.pil2hip_timing_read(ph, True)
