@@ -1,6 +1,6 @@
 """Generate the golden fixtures under tests/golden/ (run in the build container only).
 
-Two files are written:
+Four files are written (the last two by the options named beside them):
 
 reference_vectors.json
     Known-answer vectors held in the reference's own unit tests, read from their
@@ -18,13 +18,18 @@ compiled_reference.json
     NTT around the regrouping of starks.hpp:209-221). Digest = sha256 of the
     canonical row-major array (little endian).
 
-config_reference.json   (--configs; about 40 min on 8 cores, 40 GB of host memory)
+config_reference.json   (--configs; about 40 min on 8 cores, 45 GB of host memory)
     The configurations at their real sizes, from the same compiled reference: the
     plain-large root, LDE digest and tree digest (2^22 x 128 -> 2^24, arity 3), the
     roots of the 64-instance mixed batch, and root + group-proof digest for the
     first 64 jobs of the serving stream (log2 rows 12..22).
 
-Usage: python tests/golden/make_fixtures.py [--configs]
+large_domain_reference.json   (--large; about 5 min, 16 GB of host memory)
+    Transforms above 2^24 points from the same compiled reference: LDE digests and
+    roots at 2^25 and 2^28, plain NTT / INTT digests at 2^25 and 2^28, and a
+    computeQ digest at NExt = 2^25.
+
+Usage: python tests/golden/make_fixtures.py [--configs | --large]
 """
 import hashlib
 import json
@@ -145,7 +150,7 @@ def compute_q_cases(ref):
     return out
 
 
-def _ref_commit(ref, orc, n_bits, n_bits_ext, n_cols, seed, arity, keep_lde=False):
+def _ref_commit(ref, orc, n_bits, n_bits_ext, n_cols, seed, arity):
     """The reference's CPU commit of one seeded trace at its full size, lean in host
     memory (the largest serving job's LDE alone is 29 GiB): trace from the C
     generator, NTT_Goldilocks::LDE straight into a buffer with the 8 rows of slack
