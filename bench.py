@@ -225,7 +225,7 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
         "perm_per_s": perms / leaf_s,
         "mul_peak_measured": round(measured_mul_peak()[0], 4),
         "frac_of_measured_mul_peak": round(muls_per_s / 1e12 / measured_mul_peak()[0], 4),
-        "mul_peak_note": "pil2hip_mul_peak: the kernel's own reduced product (13 single-rate instructions), "
+        "mul_peak_note": "pil2hip_mul_peak: the kernel's own reduced product (11 single-rate instructions), "
                          "18 independent chains per lane, 2^21 lanes",
         "hbm_achieved_GBs": leaf_bytes / leaf_s / 1e9, "hbm_frac": leaf_bytes / leaf_s / 1e9 / HBM_PEAK_GBS,
         "traffic": traffic,
@@ -233,11 +233,12 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
     }
 
 
-LDE_KERNELS = ("k_transpose_rm2cm", "k_ntt_strided_inv", "k_lde_block<false, false>", "k_ntt_strided_fwd<false>")
+LDE_KERNELS = ("k_transpose_rm2cm", "k_ntt_strided_inv<false>", "k_lde_block<false, false, false>",
+               "k_ntt_strided_fwd<false, false>")
 # single-rate VALU instructions of one radix-2 butterfly at the floor: a reduced
-# Goldilocks product (4 v_mad_u64_u32 + 9-instruction reduction) plus a 6-instruction
-# add and a 6-instruction sub, canonical in and out
-INSTR_PER_BUTTERFLY = 24
+# Goldilocks product (4 v_mad_u64_u32 + a 7-instruction reduction) plus a 6-instruction
+# add and a 6-instruction sub
+INSTR_PER_BUTTERFLY = 23
 
 
 def _lde_valu_roofline(nb, nbe, cols, lde_ms, config):
@@ -276,7 +277,7 @@ def _with_issue(roofline, config):
         roofline["valu_issue_frac"] = round(f, 4)
         roofline["valu_issue_note"] = ("VALU issue slots filled (rocprof SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2 per "
                                        "launch from profiles/, 1024 SIMDs at 2.4 GHz); the products-only 'frac' "
-                                       "counts 4 of the ~13 single-rate instructions a reduced product takes")
+                                       "counts 4 of the 11 single-rate instructions a reduced product takes")
     return roofline
 
 
