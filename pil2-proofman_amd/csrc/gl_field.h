@@ -928,10 +928,144 @@ __device__ __forceinline__ void gl_mul_v(uint64_t *out, const uint64_t *a, const
     for (int k = N / 3 * 3; k < N; k++) out[k] = gl_canon_dev(gl_mulk1(a[k], b[k], 0, 0));
 }
 
+// ---- multiplication by a power of two ------------------------------------
+// x * 2^S mod p for 0 < S < 96 and any 64-bit x, with no 64x64 product. The roots
+// of unity of order <= 64 are powers of two (w_64 = 8), so inside a 16-point DFT
+// every twiddle is w_16^k = 2^(12k) or w_16^-k = -2^(96 - 12k).
+//   S < 64:  x 2^S = L + 2^64 H with L = x << S and H = x >> (64 - S) < 2^S;
+//            2^64 = e = 2^32 - 1 and 2^96 = -1 give L + e H0 - H1.
+//   S >= 64: with t = S - 64, x 2^S = (x 2^t) 2^64 and 2^128 = -2^32 give
+//            e L0 - Q, L0 = (x << t) mod 2^32 and Q = x >> (32 - t) < 2^(32 + t).
+// Both are r = A e + lo - (B1:B0) (A, B0, B1 32-bit words, lo a 64-bit word): one
+// v_mad_u64_u32 with carry-out c, a 64-bit subtraction with borrow-out b, then
+// + e for c & !b and - e for b & !c. Neither correction wraps: c = 1 leaves
+// r < (2^32 - 1)^2 before the subtraction; b = 1 with c = 0 leaves r >= 2^64 - 2^32
+// (S < 64, where B1 = 0) or r >= 2^64 - 2^(32 + t) (S >= 64, where c = 0).
+template <int S>
+__device__ __forceinline__ void gl_pow2_terms(uint64_t x, uint32_t &A, uint64_t &lo, uint32_t &B0, uint32_t &B1)
+{
+    static_assert(S > 0 && S < 96, "power of two out of range");
+    if constexpr (S < 64) {
+        uint64_t h = x >> (64 - S);
+        A = (uint32_t)h;
+        lo = x << S;
+        B0 = (uint32_t)(h >> 32);
+        B1 = 0;
+    } else {
+        constexpr int t = S - 64;
+        uint64_t q = x >> (32 - t);
+        A = (uint32_t)x << t;
+        lo = 0;
+        B0 = (uint32_t)q;
+        B1 = (uint32_t)(q >> 32);
+    }
+}
+
+// Three lanes of r = A e + lo - (B1:B0) with the correction above (congruent
+// 64-bit words); every carry mask is read at least two instructions after its write.
+__device__ __forceinline__ void gl_pow2_reduce3(const uint32_t *A, const uint64_t *lo, const uint32_t *B0,
+                                                const uint32_t *B1, uint64_t *out)
+{
+    uint64_t r[3], c[3];
+    asm("v_mad_u64_u32 %0, %3, %6, -1, %9\n\t"
+        "v_mad_u64_u32 %1, %4, %7, -1, %10\n\t"
+        "v_mad_u64_u32 %2, %5, %8, -1, %11"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&s"(c[0]), "=&s"(c[1]), "=&s"(c[2])
+        : "v"(A[0]), "v"(A[1]), "v"(A[2]), "v"(lo[0]), "v"(lo[1]), "v"(lo[2]));
+    uint32_t r0[3], r1[3], tm[3], tn[3];
+    uint64_t b[3], m[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r0[i] = (uint32_t)r[i];
+        r1[i] = (uint32_t)(r[i] >> 32);
+    }
+    asm("v_sub_co_u32_e64 %[r0a], %[ba], %[r0a], %[x0a]\n\t"
+        "v_sub_co_u32_e64 %[r0b], %[bb], %[r0b], %[x0b]\n\t"
+        "v_sub_co_u32_e64 %[r0c], %[bc], %[r0c], %[x0c]\n\t"
+        "v_subb_co_u32_e64 %[r1a], %[ba], %[r1a], %[x1a], %[ba]\n\t"
+        "v_subb_co_u32_e64 %[r1b], %[bb], %[r1b], %[x1b], %[bb]\n\t"
+        "v_subb_co_u32_e64 %[r1c], %[bc], %[r1c], %[x1c], %[bc]\n\t"
+        "s_andn2_b64 %[ma], %[ca], %[ba]\n\t"
+        "s_andn2_b64 %[mb], %[cb], %[bb]\n\t"
+        "s_andn2_b64 %[mc], %[cc], %[bc]\n\t"
+        "s_andn2_b64 %[ba], %[ba], %[ca]\n\t"
+        "s_andn2_b64 %[bb], %[bb], %[cb]\n\t"
+        "s_andn2_b64 %[bc], %[bc], %[cc]\n\t"
+        "v_cndmask_b32_e64 %[tma], 0, -1, %[ma]\n\t"
+        "v_cndmask_b32_e64 %[tmb], 0, -1, %[mb]\n\t"
+        "v_cndmask_b32_e64 %[tmc], 0, -1, %[mc]\n\t"
+        "v_cndmask_b32_e64 %[tna], 0, -1, %[ba]\n\t"
+        "v_cndmask_b32_e64 %[tnb], 0, -1, %[bb]\n\t"
+        "v_cndmask_b32_e64 %[tnc], 0, -1, %[bc]\n\t"
+        "v_addc_co_u32_e64 %[r0a], %[ca], %[r0a], %[tma], %[ba]\n\t"
+        "v_addc_co_u32_e64 %[r0b], %[cb], %[r0b], %[tmb], %[bb]\n\t"
+        "v_addc_co_u32_e64 %[r0c], %[cc], %[r0c], %[tmc], %[bc]\n\t"
+        "v_addc_co_u32_e64 %[r1a], %[ca], %[r1a], %[tna], %[ca]\n\t"
+        "v_addc_co_u32_e64 %[r1b], %[cb], %[r1b], %[tnb], %[cb]\n\t"
+        "v_addc_co_u32_e64 %[r1c], %[cc], %[r1c], %[tnc], %[cc]"
+        : [r0a] "+v"(r0[0]), [r0b] "+v"(r0[1]), [r0c] "+v"(r0[2]), [r1a] "+v"(r1[0]), [r1b] "+v"(r1[1]),
+          [r1c] "+v"(r1[2]), [tma] "=&v"(tm[0]), [tmb] "=&v"(tm[1]), [tmc] "=&v"(tm[2]), [tna] "=&v"(tn[0]),
+          [tnb] "=&v"(tn[1]), [tnc] "=&v"(tn[2]), [ba] "=&s"(b[0]), [bb] "=&s"(b[1]), [bc] "=&s"(b[2]),
+          [ma] "=&s"(m[0]), [mb] "=&s"(m[1]), [mc] "=&s"(m[2]), [ca] "+s"(c[0]), [cb] "+s"(c[1]), [cc] "+s"(c[2])
+        : [x0a] "v"(B0[0]), [x0b] "v"(B0[1]), [x0c] "v"(B0[2]), [x1a] "v"(B1[0]), [x1b] "v"(B1[1]),
+          [x1c] "v"(B1[2])
+        : "scc");
+#pragma unroll
+    for (int i = 0; i < 3; i++) out[i] = gl_pack(r0[i], r1[i]);
+}
+
+// One lane of gl_pow2_reduce3, hazard gaps filled with s_nop.
+__device__ __forceinline__ uint64_t gl_pow2_reduce1(uint32_t A, uint64_t lo, uint32_t B0, uint32_t B1)
+{
+    uint64_t r, c, bm, m;
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(A), "v"(lo));
+    uint32_t r0 = (uint32_t)r, r1 = (uint32_t)(r >> 32), tm, tn;
+    asm("v_sub_co_u32_e64 %[r0], %[b], %[r0], %[x0]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[r1], %[b], %[r1], %[x1], %[b]\n\t"
+        "s_nop 1\n\t"
+        "s_andn2_b64 %[m], %[c], %[b]\n\t"
+        "s_andn2_b64 %[b], %[b], %[c]\n\t"
+        "v_cndmask_b32_e64 %[tm], 0, -1, %[m]\n\t"
+        "v_cndmask_b32_e64 %[tn], 0, -1, %[b]\n\t"
+        "v_addc_co_u32_e64 %[r0], %[c], %[r0], %[tm], %[b]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %[r1], %[c], %[r1], %[tn], %[c]"
+        : [r0] "+v"(r0), [r1] "+v"(r1), [tm] "=&v"(tm), [tn] "=&v"(tn), [b] "=&s"(bm), [m] "=&s"(m), [c] "+s"(c)
+        : [x0] "v"(B0), [x1] "v"(B1)
+        : "scc");
+    return gl_pack(r0, r1);
+}
+
+// canonical out[k] = x[k] * 2^S_k for three lanes (out may alias x)
+template <int S0, int S1, int S2>
+__device__ __forceinline__ void gl_mul_pow2_3(const uint64_t *x, uint64_t *out)
+{
+    uint32_t A[3], B0[3], B1[3];
+    uint64_t lo[3];
+    gl_pow2_terms<S0>(x[0], A[0], lo[0], B0[0], B1[0]);
+    gl_pow2_terms<S1>(x[1], A[1], lo[1], B0[1], B1[1]);
+    gl_pow2_terms<S2>(x[2], A[2], lo[2], B0[2], B1[2]);
+    gl_pow2_reduce3(A, lo, B0, B1, out);
+    gl_canon3(out);
+}
+
+// canonical x * 2^S
+template <int S>
+__device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x)
+{
+    uint32_t A, B0, B1;
+    uint64_t lo;
+    gl_pow2_terms<S>(x, A, lo, B0, B1);
+    return gl_canon_dev(gl_pow2_reduce1(A, lo, B0, B1));
+}
+
 #else  // host pass: declarations only (kernels are compiled for the device)
 template <int N> __device__ void gl_add_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
 template <int N> __device__ void gl_sub_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
 template <int N> __device__ void gl_mul_v(uint64_t *out, const uint64_t *a, const uint64_t *b);
+template <int S0, int S1, int S2> __device__ void gl_mul_pow2_3(const uint64_t *x, uint64_t *out);
+template <int S> __device__ uint64_t gl_mul_pow2(uint64_t x);
 #endif
 
 // a, b canonical -> canonical a + b

@@ -57,11 +57,10 @@ static NttTables tables_for(const NttTablesHost *th, uint32_t log_max)
 __device__ __forceinline__ uint32_t tw_lo(uint32_t x, const NttTables &tb) { return x & ((1u << tb.bits) - 1u); }
 __device__ __forceinline__ uint32_t tw_hi(uint32_t x, const NttTables &tb) { return x >> tb.bits; }
 
-// w_16^k and w_16^-k for k < 8 (w_16 = W[4] = 2^12)
-__constant__ uint64_t c_w16f[8];
-__constant__ uint64_t c_w16i[8];
-
 __device__ __forceinline__ uint32_t lds_addr(uint32_t x) { return x + (x >> 4); }
+// For tile indices with disjoint bits, lds_addr(a | b) = lds_addr(a) + lds_addr(b): a
+// round's values sit at lds_addr(base) + lds_addr(j << shift), the second term the
+// same for every lane (a scalar offset).
 
 // ------------------------------------------------------------ transpose ----
 // in: rows x cols row-major (row stride in_rs), out: column c at out + c*out_cs.
@@ -114,32 +113,56 @@ __device__ constexpr int tw_k(int m)
     return (m / (H - 1)) * H + m % (H - 1) + 1;
 }
 
-// One DIF stage on R values in registers: (u, x) -> (u + x, (u - x) * w); the
-// additions, subtractions and non-trivial products each run lane-interleaved.
+// The internal twiddles are powers of w_16 = 2^12 (W[4] of the reference's root
+// chain): w_16^k = 2^(12k) and w_16^-k = -2^(96 - 12k), since 2^96 = -1. They are
+// applied as multiplications by a power of two (gl_mul_pow2), and the sign of
+// the inverse roots is folded into the stage's additions and subtractions.
+template <bool INV>
+__device__ constexpr int w16_shift(int k)
+{
+    return INV ? 96 - 12 * k : 12 * k;
+}
+
+// v[b] *= 2^(shift of butterfly b's twiddle) for the butterflies b = tw_k<H>(m),
+// m in [M, NM), three lanes at a time
+template <int H, bool INV, int M, int NM>
+__device__ __forceinline__ void stage_twiddles(uint64_t *v)
+{
+    if constexpr (M + 3 <= NM) {
+        constexpr int b0 = tw_k<H>(M), b1 = tw_k<H>(M + 1), b2 = tw_k<H>(M + 2);
+        uint64_t t[3] = {v[b0], v[b1], v[b2]};
+        gl_mul_pow2_3<w16_shift<INV>(b0 % H * 8 / H), w16_shift<INV>(b1 % H * 8 / H),
+                      w16_shift<INV>(b2 % H * 8 / H)>(t, t);
+        v[b0] = t[0];
+        v[b1] = t[1];
+        v[b2] = t[2];
+        stage_twiddles<H, INV, M + 3, NM>(v);
+    } else if constexpr (M < NM) {
+        constexpr int b0 = tw_k<H>(M);
+        v[b0] = gl_mul_pow2<w16_shift<INV>(b0 % H * 8 / H)>(v[b0]);
+        stage_twiddles<H, INV, M + 1, NM>(v);
+    }
+}
+
+// One DIF stage on R values in registers: (u, x) -> (u + x, (u - x) * w); with an
+// inverse root w = -2^s the difference is taken as x - u. The additions,
+// subtractions and twiddle products each run lane-interleaved.
 template <int R, bool INV, int ST>
 __device__ __forceinline__ void dif_stage(uint64_t *v)
 {
     constexpr int H = 1 << ST, NB = R / 2;
-    uint64_t u[NB], x[NB], sm[NB], df[NB];
+    uint64_t u[NB], x[NB], sm[NB], df[NB], da[NB], db[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         u[k] = v[(k / H) * 2 * H + k % H];
         x[k] = v[(k / H) * 2 * H + k % H + H];
+        bool neg = INV && (k % H) != 0;
+        da[k] = neg ? x[k] : u[k];
+        db[k] = neg ? u[k] : x[k];
     }
     gl_add_v<NB>(sm, u, x);
-    gl_sub_v<NB>(df, u, x);
-    if constexpr (H > 1) {
-        constexpr int NM = NB / H * (H - 1);
-        uint64_t mi[NM], mw[NM], mo[NM];
-#pragma unroll
-        for (int m = 0; m < NM; m++) {
-            mi[m] = df[tw_k<H>(m)];
-            mw[m] = INV ? c_w16i[tw_k<H>(m) % H * 8 / H] : c_w16f[tw_k<H>(m) % H * 8 / H];
-        }
-        gl_mul_v<NM>(mo, mi, mw);
-#pragma unroll
-        for (int m = 0; m < NM; m++) df[tw_k<H>(m)] = mo[m];
-    }
+    gl_sub_v<NB>(df, da, db);
+    if constexpr (H > 1) stage_twiddles<H, INV, 0, NB / H * (H - 1)>(df);
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         v[(k / H) * 2 * H + k % H] = sm[k];
@@ -147,35 +170,26 @@ __device__ __forceinline__ void dif_stage(uint64_t *v)
     }
 }
 
-// One DIT stage: (u, x) -> (u + x*w, u - x*w).
+// One DIT stage: (u, x) -> (u + x*w, u - x*w); with an inverse root w = -2^s,
+// t = x*2^s gives (u - t, u + t).
 template <int R, bool INV, int ST>
 __device__ __forceinline__ void dit_stage(uint64_t *v)
 {
     constexpr int H = 1 << ST, NB = R / 2;
-    uint64_t u[NB], x[NB], sm[NB], df[NB];
+    uint64_t u[NB], x[NB], sa[NB], sd[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         u[k] = v[(k / H) * 2 * H + k % H];
         x[k] = v[(k / H) * 2 * H + k % H + H];
     }
-    if constexpr (H > 1) {
-        constexpr int NM = NB / H * (H - 1);
-        uint64_t mi[NM], mw[NM], mo[NM];
-#pragma unroll
-        for (int m = 0; m < NM; m++) {
-            mi[m] = x[tw_k<H>(m)];
-            mw[m] = INV ? c_w16i[tw_k<H>(m) % H * 8 / H] : c_w16f[tw_k<H>(m) % H * 8 / H];
-        }
-        gl_mul_v<NM>(mo, mi, mw);
-#pragma unroll
-        for (int m = 0; m < NM; m++) x[tw_k<H>(m)] = mo[m];
-    }
-    gl_add_v<NB>(sm, u, x);
-    gl_sub_v<NB>(df, u, x);
+    if constexpr (H > 1) stage_twiddles<H, INV, 0, NB / H * (H - 1)>(x);
+    gl_add_v<NB>(sa, u, x);
+    gl_sub_v<NB>(sd, u, x);
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-        v[(k / H) * 2 * H + k % H] = sm[k];
-        v[(k / H) * 2 * H + k % H + H] = df[k];
+        bool neg = INV && (k % H) != 0;
+        v[(k / H) * 2 * H + k % H] = neg ? sd[k] : sa[k];
+        v[(k / H) * 2 * H + k % H + H] = neg ? sa[k] : sd[k];
     }
 }
 
@@ -264,9 +278,9 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
-        uint64_t v[R];
+        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii), v[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
+        for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << (q + lt))];
         dft_dif<R, true>(v);
         uint32_t og = global_o<SPLIT>(p.o_lo, qlo, m_lo);
         if (og) {
@@ -277,7 +291,7 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
             twiddle_rest<R>(v, w);
         }
 #pragma unroll
-        for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
+        for (int j = 0; j < R; j++) sb[lds_addr((uint32_t)j << (q + lt))] = v[j];
     }
 }
 
@@ -293,9 +307,9 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
-        uint64_t v[R];
+        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii), v[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) v[j] = s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)];
+        for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << (q + lt))];
         uint32_t og = global_o<SPLIT>(p.o_lo, qlo, m_lo);
         if (og) {
             uint64_t w[R];
@@ -306,7 +320,7 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
         }
         dft_dit<R, INV>(v);
 #pragma unroll
-        for (int j = 0; j < R; j++) s[lds_addr((round_m(p, j, q, LR) << lt) | p.ii)] = v[j];
+        for (int j = 0; j < R; j++) sb[lds_addr((uint32_t)j << (q + lt))] = v[j];
     }
 }
 
@@ -455,9 +469,9 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * ((1u << L2) >> Log2<V>::v) + t, 0, q);
-        uint64_t v[R];
+        uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR)), v[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) v[j] = s[lds_addr(round_m(p, j, q, LR))];
+        for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << q)];
         dft_dif<R, true>(v);
         if (p.o_lo) {
             uint64_t w[R];
@@ -479,7 +493,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
             gl_mul_v<R>(v, v, lo);
         }
 #pragma unroll
-        for (int j = 0; j < R; j++) s[lds_addr(round_m(p, j, q, LR))] = v[j];
+        for (int j = 0; j < R; j++) sb[lds_addr((uint32_t)j << q)] = v[j];
     }
 }
 
@@ -523,11 +537,11 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * blockDim.x + t, 0, q);
+        // the first round (q = logb) reads position P >> logb = (P_0 >> logb) | j
+        uint32_t sh = first ? q - logb : q;
+        const uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR) >> (first ? logb : 0));
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            uint32_t P = round_m(p, j, q, LR);
-            v[gg][j] = first ? s[lds_addr(P >> logb)] : s[lds_addr(P)];
-        }
+        for (int j = 0; j < R; j++) v[gg][j] = sb[lds_addr((uint32_t)j << sh)];
     }
     if (first) __syncthreads();  // the spread overwrites coefficients other threads still read
 #pragma unroll
@@ -572,8 +586,9 @@ __device__ __forceinline__ void block_dit_round(uint64_t *__restrict__ s, uint64
 #pragma unroll
             for (int j = 0; j < R; j++) dst[round_m(p, j, q, LR)] = v[gg][j];
         } else {
+            uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR));
 #pragma unroll
-            for (int j = 0; j < R; j++) s[lds_addr(round_m(p, j, q, LR))] = v[gg][j];
+            for (int j = 0; j < R; j++) sb[lds_addr((uint32_t)j << q)] = v[gg][j];
         }
     }
 }
@@ -790,7 +805,7 @@ __global__ __launch_bounds__(256) void k_ntt_tiny(const uint64_t *src, uint64_t 
 }
 
 // ------------------------------------------------------------ launchers ----
-static const uint64_t kW[33] = {
+static constexpr uint64_t kW[33] = {
     0x1ULL, 18446744069414584320ULL, 281474976710656ULL, 16777216ULL, 4096ULL, 64ULL, 8ULL,
     2198989700608ULL, 4404853092538523347ULL, 6434636298004421797ULL, 4255134452441852017ULL,
     9113133275150391358ULL, 4355325209153869931ULL, 4308460244895131701ULL,
@@ -806,18 +821,10 @@ static bool g_attr_done = false;
 
 extern "C" int ntt_init_device(void)
 {
-    // inverse 16th roots for the register DFTs, and > 64 KiB of dynamic LDS
-    uint64_t w16f[8], w16i[8];
-    uint64_t wi = gl_inv(kW[4]), a = 1, b = 1;
-    for (int k = 0; k < 8; k++) {
-        w16f[k] = a;
-        w16i[k] = b;
-        a = gl_mul(a, kW[4]);
-        b = gl_mul(b, wi);
-    }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_w16f), w16f, sizeof(w16f));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_w16i), w16i, sizeof(w16i));
-    if (e != hipSuccess) return (int)e;
+    // the register DFTs multiply by powers of two: w_16 must be 2^12
+    static_assert(kW[4] == 4096ull, "w_16 of the root chain is 2^12");
+    hipError_t e = hipSuccess;
+    // > 64 KiB of dynamic LDS
     if (!g_attr_done) {
         int bytes = (int)(NTT_LDS_WORDS * 8);
         const void *strided[] = {(const void *)k_ntt_strided_inv<false>, (const void *)k_ntt_strided_inv<true>,
