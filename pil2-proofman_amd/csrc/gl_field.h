@@ -168,7 +168,10 @@ struct gl_prodk {
 };
 
 // Three lanes; the carry masks are written by one asm block, so every later
-// read is at least two instructions after its write.
+// read is at least two instructions after its write. BU: b is wave-uniform (held
+// in SGPRs, e.g. a constant of the permutation), so the asm takes it as a scalar
+// operand instead of a copy in a VGPR.
+template <bool BU = false>
 __device__ __forceinline__ void gl_prodk3(const uint64_t *a, const uint64_t *b, uint32_t s_lo, uint32_t s_hi,
                                           gl_prodk *x)
 {
@@ -179,12 +182,20 @@ __device__ __forceinline__ void gl_prodk3(const uint64_t *a, const uint64_t *b, 
         p[i] = (uint64_t)a0 * b0 + s_lo;
         t[i] = (uint64_t)a0 * b1 + ((p[i] >> 32) + (uint64_t)s_hi);
     }
-    asm("v_mad_u64_u32 %0, %3, %6, %9, %12\n\t"
-        "v_mad_u64_u32 %1, %4, %7, %10, %13\n\t"
-        "v_mad_u64_u32 %2, %5, %8, %11, %14"
-        : "=&v"(u[0]), "=&v"(u[1]), "=&v"(u[2]), "=&s"(x[0].k), "=&s"(x[1].k), "=&s"(x[2].k)
-        : "v"((uint32_t)(a[0] >> 32)), "v"((uint32_t)(a[1] >> 32)), "v"((uint32_t)(a[2] >> 32)), "v"((uint32_t)b[0]),
-          "v"((uint32_t)b[1]), "v"((uint32_t)b[2]), "v"(t[0]), "v"(t[1]), "v"(t[2]));
+    if constexpr (BU)
+        asm("v_mad_u64_u32 %0, %3, %6, %9, %12\n\t"
+            "v_mad_u64_u32 %1, %4, %7, %10, %13\n\t"
+            "v_mad_u64_u32 %2, %5, %8, %11, %14"
+            : "=&v"(u[0]), "=&v"(u[1]), "=&v"(u[2]), "=&s"(x[0].k), "=&s"(x[1].k), "=&s"(x[2].k)
+            : "v"((uint32_t)(a[0] >> 32)), "v"((uint32_t)(a[1] >> 32)), "v"((uint32_t)(a[2] >> 32)),
+              "s"((uint32_t)b[0]), "s"((uint32_t)b[1]), "s"((uint32_t)b[2]), "v"(t[0]), "v"(t[1]), "v"(t[2]));
+    else
+        asm("v_mad_u64_u32 %0, %3, %6, %9, %12\n\t"
+            "v_mad_u64_u32 %1, %4, %7, %10, %13\n\t"
+            "v_mad_u64_u32 %2, %5, %8, %11, %14"
+            : "=&v"(u[0]), "=&v"(u[1]), "=&v"(u[2]), "=&s"(x[0].k), "=&s"(x[1].k), "=&s"(x[2].k)
+            : "v"((uint32_t)(a[0] >> 32)), "v"((uint32_t)(a[1] >> 32)), "v"((uint32_t)(a[2] >> 32)),
+              "v"((uint32_t)b[0]), "v"((uint32_t)b[1]), "v"((uint32_t)b[2]), "v"(t[0]), "v"(t[1]), "v"(t[2]));
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         x[i].h = (uint64_t)(uint32_t)(a[i] >> 32) * (uint32_t)(b[i] >> 32) + (u[i] >> 32);
@@ -254,13 +265,17 @@ __device__ __forceinline__ void gl_reducek3(const gl_prodk *x, uint64_t *out)
 }
 
 // One lane of gl_prodk3 / gl_reducek3, hazard gaps filled with s_nop.
+template <bool BU = false>
 __device__ __forceinline__ uint64_t gl_mulk1(uint64_t a, uint64_t b, uint32_t s_lo, uint32_t s_hi)
 {
     uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
     uint64_t p = (uint64_t)a0 * b0 + s_lo;
     uint64_t t = (uint64_t)a0 * b1 + ((p >> 32) + (uint64_t)s_hi);
     uint64_t u, k;
-    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=&v"(u), "=&s"(k) : "v"(a1), "v"(b0), "v"(t));
+    if constexpr (BU)
+        asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=&v"(u), "=&s"(k) : "v"(a1), "s"(b0), "v"(t));
+    else
+        asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=&v"(u), "=&s"(k) : "v"(a1), "v"(b0), "v"(t));
     uint64_t h = (uint64_t)a1 * b1 + (u >> 32);
     uint64_t lo = gl_pack((uint32_t)p, (uint32_t)u), r, c, bm, m;
     asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"((uint32_t)h), "v"(lo));

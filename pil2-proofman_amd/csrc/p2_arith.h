@@ -148,6 +148,12 @@ __device__ __forceinline__ uint64_t p2_mul_add(uint64_t a, uint64_t b, uint32_t 
     return gl_mulk1(a, b, s_lo, s_hi);
 }
 
+// p2_mul_add with a wave-uniform b
+__device__ __forceinline__ uint64_t p2_mul_add_u(uint64_t a, uint64_t b, uint32_t s_lo, uint32_t s_hi)
+{
+    return gl_mulk1<true>(a, b, s_lo, s_hi);
+}
+
 // out[k] = a[k]*b[k] mod p for three lanes (any 64-bit words)
 __device__ __forceinline__ void p2_mul3(const uint64_t *a, const uint64_t *b, uint64_t *out)
 {
@@ -156,12 +162,12 @@ __device__ __forceinline__ void p2_mul3(const uint64_t *a, const uint64_t *b, ui
     gl_reducek3(x, out);
 }
 
-// out[k] = a[k]*b[k] + s mod p for three lanes (s = s_hi:s_lo shared)
+// out[k] = a[k]*b[k] + s mod p for three lanes (s = s_hi:s_lo shared; b wave-uniform)
 __device__ __forceinline__ void p2_mul_add3(const uint64_t *a, const uint64_t *b, uint32_t s_lo, uint32_t s_hi,
                                             uint64_t *out)
 {
     gl_prodk x[3];
-    gl_prodk3(a, b, s_lo, s_hi, x);
+    gl_prodk3<true>(a, b, s_lo, s_hi, x);
     gl_reducek3(x, out);
 }
 
@@ -175,5 +181,6 @@ __device__ uint64_t hz_fold(hz a);
 __device__ uint64_t lz_fold(lz a);
 __device__ uint64_t p2_mul(uint64_t a, uint64_t b);
 __device__ uint64_t p2_mul_add(uint64_t a, uint64_t b, uint32_t s_lo, uint32_t s_hi);
+__device__ uint64_t p2_mul_add_u(uint64_t a, uint64_t b, uint32_t s_lo, uint32_t s_hi);
 
 #endif

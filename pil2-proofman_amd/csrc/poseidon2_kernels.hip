@@ -215,7 +215,7 @@ __device__ __forceinline__ void p2_permute(uint64_t *s)
 #pragma unroll
         for (int i = 0; i < G; i += 3) p2_mul_add3(s + i, d + i, s_lo, s_hi, s + i);
 #pragma unroll
-        for (int i = G; i < W; i++) s[i] = p2_mul_add(s[i], d[i], s_lo, s_hi);
+        for (int i = G; i < W; i++) s[i] = p2_mul_add_u(s[i], d[i], s_lo, s_hi);
     }
 #pragma unroll
     for (int i = 0; i < W; i++) t[i] = {(uint32_t)s[i], s[i] >> 32};
