@@ -334,11 +334,16 @@ __device__ __forceinline__ uint32_t strided_tile(uint32_t bx, uint32_t lt)
     return (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u);
 }
 
+// CANON: the words may be non-canonical (a pass over caller data); the forward
+// pass reads the block pass's canonical output and loads it as is.
+template <bool CANON>
 __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ col, uint64_t n2,
                                              uint64_t i0, uint32_t lt, uint32_t tile)
 {
-    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
-        s[lds_addr(x)] = gl_canon(col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))]);
+    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
+        uint64_t v = col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))];
+        s[lds_addr(x)] = CANON ? gl_canon(v) : v;
+    }
     __syncthreads();
 }
 
@@ -378,7 +383,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
     uint32_t tile = 1u << (L + lt);
-    strided_load(s, col, B.stride, 0, lt, tile);
+    strided_load<true>(s, col, B.stride, 0, lt, tile);
     // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
     // strided tile keeps 16 values per thread whatever the plan's radix
     int q = (int)L, nr = ((int)L + 3) / 4, kr = 0;
@@ -435,7 +440,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
     uint32_t tile = 1u << (L + lt);
-    strided_load(s, col, B.stride, 0, lt, tile);
+    strided_load<false>(s, col, B.stride, 0, lt, tile);
     uint32_t q = 0, nr = (L + 3) / 4, kr = 0;  // rounds as in k_ntt_strided_inv
     while (q < L) {
         uint32_t r = ((L - q) + (nr - kr) - 1) / (nr - kr);
