@@ -1242,28 +1242,77 @@ struct FriGraph {
 static std::vector<FriGraph> g_fri_graphs;
 static std::mutex g_fri_mu;
 static hipStream_t g_fri_capture = nullptr;
+// Graph branches for the layer trees: the challenges are all in device memory
+// before the sequence starts, so a layer's tree (leaves and levels, on its own
+// source and node buffers) depends only on that layer's transpose, and the next
+// fold only on the transpose having read d_work. The trees of different layers
+// then run side by side instead of one after another; each is a chain of
+// latency-bound levels.
+static constexpr uint32_t kFriBranches = 4;
+static hipStream_t g_fri_side[kFriBranches];
+static std::vector<hipEvent_t> g_fri_fork;         // one per layer
+static hipEvent_t g_fri_join[kFriBranches];
 
+static int64_t fri_branch_resources(uint64_t layers)
+{
+    if (!g_fri_side[0]) {
+        for (uint32_t b = 0; b < kFriBranches; b++) {
+            HIP_TRY(hipStreamCreateWithFlags(&g_fri_side[b], hipStreamNonBlocking), "FRI branch stream");
+            HIP_TRY(hipEventCreateWithFlags(&g_fri_join[b], hipEventDisableTiming), "FRI join event");
+        }
+    }
+    while (g_fri_fork.size() < layers) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "FRI fork event");
+        g_fri_fork.push_back(e);
+    }
+    return 0;
+}
+
+// branches: each layer's tree goes on branch stream (layer mod kFriBranches),
+// forked from st after the layer's transpose and joined into st at the end (used
+// under graph capture, where the fork and join become graph edges).
 static int64_t fri_enqueue(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_ch, uint64_t nbe,
                            const uint64_t *steps, uint64_t n, uint64_t arity, uint64_t *const *d_sources,
-                           uint64_t *const *d_nodes, hipStream_t st)
+                           uint64_t *const *d_nodes, hipStream_t st, bool branches)
 {
     uint32_t width = (uint32_t)(4 * arity);
-    for (uint64_t s = 0; s < n; s++) {
+    bool used[kFriBranches] = {};
+    int64_t rc = 0;
+    for (uint64_t s = 0; s < n && !rc; s++) {
         if (s > 0) {
             int e = fri_launch_fold2(d_work, s == 1 ? d_pol : d_work, nullptr, d_ch + 3 * s, nbe, steps[s - 1],
                                      steps[s], st);
-            if (e) return fail(-100 - e, "fold launch failed");
+            if (e) rc = fail(-100 - e, "fold launch failed");
         }
-        if (s + 1 < n) {
+        if (!rc && s + 1 < n) {
             uint64_t rows = 1ull << steps[s + 1], cols = 3ull << (steps[s] - steps[s + 1]);
             int e = fri_launch_transpose(d_sources[s], s == 0 ? d_pol : d_work, 1ull << steps[s],
                                          (uint32_t)steps[s + 1], st);
-            if (!e) e = p2_launch_leaves(width, d_sources[s], cols, 1, cols, rows, d_nodes[s], st);
-            if (!e) e = p2_launch_tree_levels(width, d_nodes[s], rows, P2_COOP_MAX_PARENTS, st);
-            if (e) return fail(-100 - e, "layer commit launch failed");
+            hipStream_t ts = st;
+            if (!e && branches) {
+                uint32_t b = (uint32_t)(s % kFriBranches);
+                hipError_t fe = hipEventRecord(g_fri_fork[s], st);
+                if (fe == hipSuccess) fe = hipStreamWaitEvent(g_fri_side[b], g_fri_fork[s], 0);
+                if (fe != hipSuccess) {
+                    rc = fail(-100 - (int64_t)fe, std::string("FRI fork: ") + hipGetErrorString(fe));
+                    break;
+                }
+                ts = g_fri_side[b];
+                used[b] = true;
+            }
+            if (!e) e = p2_launch_leaves(width, d_sources[s], cols, 1, cols, rows, d_nodes[s], ts);
+            if (!e) e = p2_launch_tree_levels(width, d_nodes[s], rows, P2_COOP_MAX_PARENTS, ts);
+            if (e) rc = fail(-100 - e, "layer commit launch failed");
         }
     }
-    return 0;
+    for (uint32_t b = 0; b < kFriBranches; b++) {  // join every forked branch, also after an error
+        if (!used[b]) continue;
+        hipError_t je = hipEventRecord(g_fri_join[b], g_fri_side[b]);
+        if (je == hipSuccess) je = hipStreamWaitEvent(st, g_fri_join[b], 0);
+        if (je != hipSuccess && !rc) rc = fail(-100 - (int64_t)je, std::string("FRI join: ") + hipGetErrorString(je));
+    }
+    return rc;
 }
 
 int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_challenges, uint64_t nBitsExt,
@@ -1283,7 +1332,8 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
     for (uint64_t s = 0; s + 1 < nSteps; s++)
         if (!d_sources[s] || !d_nodes[s]) return fail(-13, "null layer buffer");
     hipStream_t st = (hipStream_t)stream;
-    if (!graph) return fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, st);
+    if (!graph)
+        return fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, st, false);
     std::vector<uint64_t> key = {(uint64_t)(uintptr_t)d_pol, (uint64_t)(uintptr_t)d_work,
                                  (uint64_t)(uintptr_t)d_challenges, nBitsExt, arity};
     for (uint64_t s = 0; s < nSteps; s++) key.push_back(steps[s]);
@@ -1298,9 +1348,11 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
             return 0;
         }
     if (!g_fri_capture) HIP_TRY(hipStreamCreateWithFlags(&g_fri_capture, hipStreamNonBlocking), "FRI capture stream");
+    if ((rc = fri_branch_resources(nSteps))) return rc;
     HIP_TRY(hipStreamBeginCapture(g_fri_capture, hipStreamCaptureModeThreadLocal), "FRI begin capture");
     t_capturing = true;
-    rc = fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, g_fri_capture);
+    rc = fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, g_fri_capture,
+                     true);
     t_capturing = false;
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(g_fri_capture, &g);
