@@ -278,7 +278,8 @@ __device__ __forceinline__ void strided_dif_round(uint64_t *__restrict__ s, uint
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
-        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii), v[R];
+        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii);
+        uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << (q + lt))];
         dft_dif<R, true>(v);
@@ -307,7 +308,8 @@ __device__ __forceinline__ void strided_dit_round(uint64_t *__restrict__ s, uint
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * blockDim.x + t, lt, q);
-        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii), v[R];
+        uint64_t *sb = s + lds_addr((round_m(p, 0, q, LR) << lt) | p.ii);
+        uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << (q + lt))];
         uint32_t og = global_o<SPLIT>(p.o_lo, qlo, m_lo);
@@ -474,7 +476,8 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
         RoundPos p = round_pos(gg * ((1u << L2) >> Log2<V>::v) + t, 0, q);
-        uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR)), v[R];
+        uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR));
+        uint64_t v[R];
 #pragma unroll
         for (int j = 0; j < R; j++) v[j] = sb[lds_addr((uint32_t)j << q)];
         dft_dif<R, true>(v);
