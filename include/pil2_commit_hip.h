@@ -420,8 +420,11 @@ int64_t pil2hip_fri_transpose(uint64_t *d_aux, const uint64_t *d_pol, uint64_t d
  * (stark_struct.rs:162-166 gives the schedule). graph = true captures the whole
  * sequence once per argument set (all pointers, steps, arity) and replays the
  * cached graph on later calls, so new challenges written into d_challenges are
- * used without a new capture; graph = false enqueues the kernels directly (for
- * a caller's own capture). d_work needs 3*2^steps[1] words. Asynchronous on
+ * used without a new capture; in the graph every layer's tree is a branch that
+ * depends only on that layer's transpose, so the trees of different layers run
+ * side by side and the roots are in d_nodes when the launch completes on
+ * `stream`. graph = false enqueues the kernels directly on `stream`, one after
+ * another (for a caller's own capture). d_work needs 3*2^steps[1] words. Asynchronous on
  * `stream`. No counterpart in the reference API: gen_proof (starks_api.hpp:122)
  * runs this loop internally. */
 int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64_t *d_challenges, uint64_t nBitsExt,
