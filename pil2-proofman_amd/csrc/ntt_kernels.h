@@ -53,3 +53,6 @@ extern "C" int ntt_launch_cm2rm(uint64_t *out, const uint64_t *in, uint64_t rows
 extern "C" int ntt_launch_rm2cm(uint64_t *out, const uint64_t *in, uint64_t rows, uint64_t cols, uint64_t in_rs,
                                 uint64_t out_cs, hipStream_t st);
 extern "C" int ntt_launch_fill_uniform(uint64_t *out, uint64_t n, uint64_t seed, uint64_t start, hipStream_t st);
+// columns [0, w) of `rows` host rows (stride ld words, page-locked memory) -> dst, rows x w row-major
+extern "C" int ntt_launch_gather_rows(uint64_t *dst, const uint64_t *src, uint64_t rows, uint32_t w, uint64_t ld,
+                                      hipStream_t st);

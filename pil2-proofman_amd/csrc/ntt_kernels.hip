@@ -1150,6 +1150,41 @@ __global__ __launch_bounds__(256) void k_fill_uniform(uint64_t *__restrict__ out
     }
 }
 
+// Rows of a column group read from page-locked host memory by the kernel itself
+// (over PCIe): dst (rows x w, row-major, device) <- columns [0, w) of the host
+// rows at src (row stride ld words). 16-byte loads and stores when every piece
+// is 16-byte aligned, 8-byte ones otherwise.
+__global__ __launch_bounds__(256) void k_gather_rows(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src,
+                                                     uint64_t rows, uint32_t w, uint64_t ld, uint32_t wide)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (wide) {
+        uint32_t cpr = w / 2;  // 16-byte pieces per row
+        uint64_t total = rows * cpr;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+            uint64_t r = i / cpr, c = (i - r * cpr) * 2;
+            *(uint4 *)(dst + r * w + c) = *(const uint4 *)(src + r * ld + c);
+        }
+    } else {
+        uint64_t total = rows * w;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+            uint64_t r = i / w, c = i - r * w;
+            dst[r * w + c] = src[r * ld + c];
+        }
+    }
+}
+
+extern "C" int ntt_launch_gather_rows(uint64_t *dst, const uint64_t *src, uint64_t rows, uint32_t w, uint64_t ld,
+                                      hipStream_t st)
+{
+    if (rows == 0 || w == 0) return 0;
+    uint32_t wide = (w % 2 == 0 && ld % 2 == 0 && ((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 15) == 0) ? 1u : 0u;
+    uint64_t items = wide ? rows * (w / 2) : rows * w, blocks = (items + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, rows, w, ld, wide);
+    return (int)hipGetLastError();
+}
+
 extern "C" int ntt_launch_fill_uniform(uint64_t *out, uint64_t n, uint64_t seed, uint64_t start, hipStream_t st)
 {
     if (n == 0) return 0;

@@ -430,15 +430,17 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
         e = p2_launch_leaves(width, d_lde, 1, NExt, nCols, NExt, d_nodes, st);
     } else {
         hipStream_t cs = nullptr;
-        hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+        hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, gathered = nullptr;
         auto done = [&](int64_t r) {
             // the host rows are free after this; d_stage stays in use until `st` has run the last group's LDE
             if (cs) (void)hipStreamSynchronize(cs);
+            if (gathered) (void)hipEventSynchronize(gathered);  // the kernel that read host rows on `st`
             if (r) (void)hipStreamSynchronize(st);   // on an error nothing queued may still read d_stage
             for (int k = 0; k < 2; k++) {
                 if (copied[k]) (void)hipEventDestroy(copied[k]);
                 if (freed[k]) (void)hipEventDestroy(freed[k]);
             }
+            if (gathered) (void)hipEventDestroy(gathered);
             if (cs) (void)hipStreamDestroy(cs);
             return r;
         };
@@ -447,13 +449,31 @@ int64_t pil2hip_commit_host(uint64_t *d_lde, uint64_t *d_nodes, const uint64_t *
             he = hipEventCreateWithFlags(&copied[k], hipEventDisableTiming);
             if (he == hipSuccess) he = hipEventCreateWithFlags(&freed[k], hipEventDisableTiming);
         }
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&gathered, hipEventDisableTiming);
         if (he != hipSuccess) return done(fail(-100 - (int64_t)he, "streamed commit: stream/event creation failed"));
+        // The first group's copy cannot overlap any compute (its LDE is the first
+        // work), so a kernel on `st`, with the device otherwise idle, reads those
+        // page-locked rows itself: 42 GB/s for 128-byte pieces against about 30 for
+        // the 2D DMA copy, while the DMA engine starts on the second group at once.
+        // (Giving the DMA engine a share of the first group was slower: both read
+        // over the same link.) Without a device pointer for h_trace the DMA copies it.
+        void *hd = nullptr;
+        bool gather = hipHostGetDevicePointer(&hd, (void *)h_trace, 0) == hipSuccess && hd;
+        (void)hipGetLastError();
+        uint64_t r_dma = gather ? 0 : N;
         uint64_t *scratch = d_stage + 2 * N * w;
         for (uint64_t g = 0, c0 = 0; c0 < nCols; g++, c0 += w) {
             uint64_t wg = std::min(w, nCols - c0), *buf = d_stage + (g & 1) * N * w;
+            uint64_t rows = g == 0 ? r_dma : N;
             if (g >= 2) he = hipStreamWaitEvent(cs, freed[g & 1], 0);  // group g-2's LDE has read buf
-            if (he == hipSuccess)
-                he = hipMemcpy2DAsync(buf, wg * 8, h_trace + c0, nCols * 8, wg * 8, N, hipMemcpyHostToDevice, cs);
+            if (he == hipSuccess && rows)
+                he = hipMemcpy2DAsync(buf, wg * 8, h_trace + c0, nCols * 8, wg * 8, rows, hipMemcpyHostToDevice, cs);
+            if (he == hipSuccess && g == 0 && rows < N) {
+                e = ntt_launch_gather_rows(buf + rows * wg, (const uint64_t *)hd + rows * nCols + c0, N - rows,
+                                           (uint32_t)wg, nCols, st);
+                if (e) return done(fail(-100 - e, "streamed commit: host-row gather launch failed"));
+                he = hipEventRecord(gathered, st);
+            }
             if (he == hipSuccess) he = hipEventRecord(copied[g & 1], cs);
             if (he == hipSuccess) he = hipStreamWaitEvent(st, copied[g & 1], 0);
             if (he != hipSuccess) return done(fail(-100 - (int64_t)he, "streamed commit: copy failed"));
