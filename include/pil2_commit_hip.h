@@ -158,9 +158,11 @@ int64_t pil2hip_commit_noalloc(uint64_t *d_lde, uint64_t *d_nodes, const uint64_
  * reference's streaming commit, stream_commit.cuh:9-20 / commit_witness_streaming,
  * starks_api.hpp:216, for an unpacked trace and Poseidon2). The row-major host
  * trace (2^nBits x nCols) is copied in groups of groupCols columns (a multiple of
- * the sponge rate 4*arity-4; 0 picks one group under 256 MiB of trace, else about
- * eight groups of at least 16 columns) on a copy stream of the
- * library's own, double-buffered, while `stream` extends the previous group into
+ * the sponge rate 4*arity-4; 0 picks one group under 256 MiB of trace, else a
+ * first group of about an eighth of the columns, at least 16, and later groups
+ * twice as wide) on a copy stream of the library's own (the first group is read
+ * from the page-locked rows by a kernel on `stream`, which the device has free
+ * then), double-buffered, while `stream` extends the previous group into
  * its columns of d_lde and absorbs it into the row sponges, whose capacity words
  * carry over in the leaf digests of d_nodes. d_lde and d_nodes end up equal to
  * pil2hip_commit's. d_stage: pil2hip_commit_host_stage_words() words. h_trace
