@@ -336,17 +336,45 @@ __device__ __forceinline__ uint32_t strided_tile(uint32_t bx, uint32_t lt)
     return (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u);
 }
 
+// A strided tile holds 16 values per thread (the launch gives it tile / 16
+// threads): thread t moves elements x = t + k * blockDim.x, k < 16. All sixteen
+// global loads are issued before the first LDS write (and all sixteen LDS reads
+// before the first global store), so a wave keeps sixteen HBM requests in flight
+// instead of waiting out one round trip per element.
 // CANON: the words may be non-canonical (a pass over caller data); the forward
 // pass reads the block pass's canonical output and loads it as is.
 template <bool CANON>
 __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ col, uint64_t n2,
-                                             uint64_t i0, uint32_t lt, uint32_t tile)
+                                             uint64_t i0, uint32_t lt)
 {
-    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x) {
-        uint64_t v = col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))];
-        s[lds_addr(x)] = CANON ? gl_canon(v) : v;
+    uint64_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        uint32_t x = threadIdx.x + k * blockDim.x;
+        v[k] = col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))];
     }
+#pragma unroll
+    for (int k = 0; k < 16; k++) s[lds_addr(threadIdx.x + k * blockDim.x)] = CANON ? gl_canon(v[k]) : v[k];
     __syncthreads();
+}
+
+// The tile back to its global positions (element x at col + (x >> lt) * stride +
+// (x & (2^lt - 1))), sixteen LDS reads first. The thread index passes through an
+// empty asm so that the compiler recomputes the sixteen addresses here instead of
+// keeping the load phase's ones live in registers across the rounds.
+__device__ __forceinline__ void strided_store(const uint64_t *__restrict__ s, uint64_t *__restrict__ col,
+                                              uint64_t stride, uint32_t lt)
+{
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    uint64_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = s[lds_addr(t + k * blockDim.x)];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        uint32_t x = t + k * blockDim.x;
+        col[(uint64_t)(x >> lt) * stride + (x & ((1u << lt) - 1))] = v[k];
+    }
 }
 
 // Which part of m (Ltot bits, global stride n2) a strided block covers: bits
@@ -385,7 +413,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
     uint32_t tile = 1u << (L + lt);
-    strided_load<true>(s, col, B.stride, 0, lt, tile);
+    strided_load<true>(s, col, B.stride, 0, lt);
     // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
     // strided tile keeps 16 values per thread whatever the plan's radix
     int q = (int)L, nr = ((int)L + 3) / 4, kr = 0;
@@ -402,8 +430,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
         __syncthreads();
     }
     if (SPLIT && !final_tw) {
-        for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
-            col[(uint64_t)(x >> lt) * B.stride + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
+        strided_store(s, col, B.stride, lt);
         return;
     }
     // store: the tile holds 16 values per thread; four at a time, twiddles interleaved
@@ -441,8 +468,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     extern __shared__ uint64_t s[];
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
-    uint32_t tile = 1u << (L + lt);
-    strided_load<false>(s, col, B.stride, 0, lt, tile);
+    strided_load<false>(s, col, B.stride, 0, lt);
     uint32_t q = 0, nr = (L + 3) / 4, kr = 0;  // rounds as in k_ntt_strided_inv
     while (q < L) {
         uint32_t r = ((L - q) + (nr - kr) - 1) / (nr - kr);
@@ -456,8 +482,37 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
         __syncthreads();
         q += r;
     }
-    for (uint32_t x = threadIdx.x; x < tile; x += blockDim.x)
-        col[(uint64_t)(x >> lt) * B.stride + (x & ((1u << lt) - 1))] = s[lds_addr(x)];
+    strided_store(s, col, B.stride, lt);
+}
+
+// A block's line of n contiguous words into LDS, canonical: thread t takes words
+// t + i * blockDim.x, i < V, with all V global loads issued before its first LDS
+// write. The block launches give n = V * blockDim.x for V >= 2 (a line of 2^L2
+// words over 2^(L2 + logb - 4) threads, V = 16 >> logb); V = 1 covers logb >= 4,
+// where a thread has at most one word.
+template <int V>
+__device__ __forceinline__ void line_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ src, uint32_t n)
+{
+    if constexpr (V == 1) {
+        if (threadIdx.x < n) s[lds_addr(threadIdx.x)] = gl_canon(src[threadIdx.x]);
+    } else {
+        uint64_t v[V];
+#pragma unroll
+        for (int i = 0; i < V; i++) v[i] = src[threadIdx.x + i * blockDim.x];
+#pragma unroll
+        for (int i = 0; i < V; i++) s[lds_addr(threadIdx.x + i * blockDim.x)] = gl_canon(v[i]);
+    }
+}
+__device__ __forceinline__ void line_load_b(uint64_t *__restrict__ s, const uint64_t *__restrict__ src, uint32_t n,
+                                            uint32_t logb)
+{
+    switch (logb) {
+    case 0: line_load<16>(s, src, n); break;
+    case 1: line_load<8>(s, src, n); break;
+    case 2: line_load<4>(s, src, n); break;
+    case 3: line_load<2>(s, src, n); break;
+    default: line_load<1>(s, src, n); break;
+    }
 }
 
 // ------------------------------------------------------ fused LDE block ----
@@ -619,7 +674,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
     const uint64_t *src = in + (uint64_t)blockIdx.y * cs_in + ((uint64_t)m << L2);
     uint32_t log_n = L1 + L2;
     // phase 1: load the line, iNTT DIF over L2 bits with 16/B values per thread
-    for (uint32_t k = threadIdx.x; k < (1u << L2); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
+    line_load_b(s, src, 1u << L2, logb);
     __syncthreads();
     if constexpr (!COEF_IN) {
         switch (logb) {
@@ -700,7 +755,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_q_block_inv(const uint64_t *__r
     uint32_t m = blockIdx.x, d = blockIdx.y;
     uint32_t lb = L2 + logb;
     const uint64_t *src = in + (uint64_t)d * cs_in + ((uint64_t)m << lb);
-    for (uint32_t k = threadIdx.x; k < (1u << lb); k += blockDim.x) s[lds_addr(k)] = gl_canon(src[k]);
+    line_load<16>(s, src, 1u << lb);
     __syncthreads();
     block_phase1<16>(s, m, lb, L1 + lb, 0, tb, false, 4);
     uint32_t n2 = 1u << L2;
