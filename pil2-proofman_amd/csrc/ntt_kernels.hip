@@ -412,7 +412,6 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     extern __shared__ uint64_t s[];
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
-    uint32_t tile = 1u << (L + lt);
     strided_load<true>(s, col, B.stride, 0, lt);
     // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
     // strided tile keeps 16 values per thread whatever the plan's radix
@@ -433,11 +432,16 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
         strided_store(s, col, B.stride, lt);
         return;
     }
-    // store: the tile holds 16 values per thread; four at a time, twiddles interleaved
+    // store: the tile holds 16 values per thread; four at a time, twiddles
+    // interleaved, in an unrolled loop so that later groups' loads can be issued early
     uint32_t Lt = SPLIT ? Ltot : L;
     uint32_t shift_n = 2 * tb.bits - (Lt + log_n2);
     uint64_t i0 = SPLIT ? B.base & ((1ull << log_n2) - 1) : B.base;
-    for (uint32_t x0 = threadIdx.x; x0 < tile; x0 += 4 * blockDim.x) {
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));  // addresses computed here, not kept live across the rounds
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        uint32_t x0 = t + it * 4 * blockDim.x;
         uint64_t v[4], lo[4], hi[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
