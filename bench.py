@@ -494,6 +494,7 @@ def run_fri(args, cfg):
     d_ch = pc.to_device(np.asarray(chal, dtype=np.uint64).reshape(-1), dev)
     for _ in range(args.warmup):
         pc.FRI.layers(src, work, d_ch, nbe, steps, trees, arity=arity, graph=True)
+    torch.cuda.synchronize()  # the warm-up's work ends before the clock starts
     profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -753,6 +754,7 @@ def run_compute_q(args, cfg):
     step = lambda: pc.compute_q(q, nb, nbe, qdeg, q_dim=qdim, arity=arity, cmq=cmq, nodes=nodes, scratch=scratch)
     for _ in range(args.warmup):
         step()
+    torch.cuda.synchronize()  # the warm-up's work ends before the clock starts
     profile_region(args, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
