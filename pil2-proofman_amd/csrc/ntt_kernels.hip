@@ -525,7 +525,10 @@ __device__ __forceinline__ void line_load_b(uint64_t *__restrict__ s, const uint
 // Phase 1: the n2 values are loaded into LDS, then DIF rounds run with V = 16/B
 // values per thread (so every thread works); the last round scales the
 // coefficient at position k (index e = bitrev_{L1+L2}(m*n2 + k)) by 7^e / N.
-template <int R, int V>
+// LINES = 2: two lines of n2 values sit one after the other in LDS (line index at
+// bit L2 of the position) and the rounds run over both; a round's bits and twiddles
+// are those of the position within its line.
+template <int R, int V, int LINES = 1>
 __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32_t q, bool last, uint32_t m,
                                                 uint32_t L2, uint32_t log_n, uint64_t n_inv, const NttTables &tb)
 {
@@ -534,7 +537,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
     uint32_t t = threadIdx.x;
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
-        RoundPos p = round_pos(gg * ((1u << L2) >> Log2<V>::v) + t, 0, q);
+        RoundPos p = round_pos(gg * (((uint32_t)LINES << L2) >> Log2<V>::v) + t, 0, q);
         uint64_t *sb = s + lds_addr(round_m(p, 0, q, LR));
         uint64_t v[R];
 #pragma unroll
@@ -550,7 +553,7 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
             uint64_t lo[R], hi[R], ni[R];
 #pragma unroll
             for (int j = 0; j < R; j++) {
-                uint32_t e = bitrev32((m << L2) | round_m(p, j, q, LR), log_n);
+                uint32_t e = bitrev32((m << L2) | (round_m(p, j, q, LR) & ((1u << L2) - 1)), log_n);
                 lo[j] = tb.shl[tw_lo(e, tb)];
                 hi[j] = tb.shh[tw_hi(e, tb)];
                 ni[j] = n_inv;
@@ -564,12 +567,12 @@ __device__ __forceinline__ void block_dif_round(uint64_t *__restrict__ s, uint32
     }
 }
 
-template <int V>
+template <int V, int LINES = 1>
 __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t m, uint32_t L2, uint32_t log_n,
                                              uint64_t n_inv, const NttTables &tb, bool scale, uint32_t rmax)
 {
     constexpr int LV = Log2<V>::v;
-    bool active = threadIdx.x < ((1u << L2) >> LV);
+    bool active = threadIdx.x < (((uint32_t)LINES << L2) >> LV);
     int rm = LV < (int)rmax ? LV : (int)rmax;
     int q = (int)L2;
     while (q > 0) {
@@ -578,10 +581,10 @@ __device__ __forceinline__ void block_phase1(uint64_t *__restrict__ s, uint32_t 
         bool last = (q == 0) && scale;
         if (active) {
             switch (r) {
-            case 4: if constexpr (V >= 16) block_dif_round<16, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
-            case 3: if constexpr (V >= 8) block_dif_round<8, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
-            case 2: if constexpr (V >= 4) block_dif_round<4, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
-            default: block_dif_round<2, V>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            case 4: if constexpr (V >= 16) block_dif_round<16, V, LINES>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            case 3: if constexpr (V >= 8) block_dif_round<8, V, LINES>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            case 2: if constexpr (V >= 4) block_dif_round<4, V, LINES>(s, q, last, m, L2, log_n, n_inv, tb); break;
+            default: block_dif_round<2, V, LINES>(s, q, last, m, L2, log_n, n_inv, tb); break;
             }
         }
         __syncthreads();
@@ -708,6 +711,70 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_block(const uint64_t *__res
         if (!last) __syncthreads();
         first = false;
         qq += r;
+    }
+}
+
+// The fused block pass for blowup 4 on two columns at once: block (m, c) takes line
+// m of columns 2c and 2c + 1. Phase 1 runs on both lines together (8 values per
+// thread instead of 4: radix-8 rounds, so half the rounds and barriers per line,
+// every thread busy). Phase 2 needs the whole tile for one line, so it runs on
+// the first line while each thread keeps four of the second line's coefficients
+// in registers; they go back to LDS for the second line's phase 2.
+__global__ __launch_bounds__(NTT_THREADS) void k_lde_block2(const uint64_t *__restrict__ in, uint64_t cs_in,
+                                                            uint64_t *__restrict__ out, uint64_t cs_out, uint32_t L1,
+                                                            uint32_t L2, uint64_t n_inv, uint32_t rmax, NttTables tb)
+{
+    constexpr uint32_t logb = 2;
+    extern __shared__ uint64_t s[];
+    const uint32_t m = blockIdx.x, n2 = 1u << L2, t = threadIdx.x, nt = blockDim.x;  // n2 = 4 * nt
+    const uint64_t c0 = 2ull * blockIdx.y;
+    const uint64_t *src = in + c0 * cs_in + ((uint64_t)m << L2);
+    {
+        uint64_t v[8];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = src[t + i * nt];
+            v[4 + i] = src[cs_in + t + i * nt];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            s[lds_addr(t + i * nt)] = gl_canon(v[i]);
+            s[lds_addr(n2 + t + i * nt)] = gl_canon(v[4 + i]);
+        }
+    }
+    __syncthreads();
+    uint32_t log_n = L1 + L2;
+    block_phase1<8, 2>(s, m, L2, log_n, n_inv, tb, true, rmax);
+    uint64_t keep[4];  // the second line's coefficients during the first line's phase 2
+#pragma unroll
+    for (int i = 0; i < 4; i++) keep[i] = s[lds_addr(n2 + t + i * nt)];
+    __syncthreads();
+    uint32_t e1 = bitrev32(m, L1);
+    uint32_t shift_e = 2 * tb.bits - (log_n + logb);
+    uint32_t top = L2 + logb;
+    for (int line = 0; line < 2; line++) {
+        if (line == 1) {
+            __syncthreads();  // every thread has read the first line's last round from LDS
+#pragma unroll
+            for (int i = 0; i < 4; i++) s[lds_addr(t + i * nt)] = keep[i];
+            __syncthreads();
+        }
+        uint64_t *dst = out + (c0 + line) * cs_out + ((uint64_t)m << top);
+        uint32_t qq = logb;
+        bool first = true;
+        while (qq < top) {
+            uint32_t r = (top - qq) >= rmax ? rmax : (top - qq);
+            bool last = (qq + r == top);
+            switch (r) {
+            case 4: block_dit_round<16, false, false>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+            case 3: block_dit_round<8, false, false>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+            case 2: block_dit_round<4, false, false>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+            default: block_dit_round<2, false, false>(s, dst, qq, logb, first, last, e1, shift_e, L1, tb); break;
+            }
+            if (!last) __syncthreads();
+            first = false;
+            qq += r;
+        }
     }
 }
 
@@ -903,7 +970,7 @@ extern "C" int ntt_init_device(void)
             if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void *)k_q_block_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        const void *blocks[] = {(const void *)k_lde_block<false, false, false>,
+        const void *blocks[] = {(const void *)k_lde_block2, (const void *)k_lde_block<false, false, false>,
                                 (const void *)k_lde_block<false, false, true>,
                                 (const void *)k_lde_block<true, false, false>,
                                 (const void *)k_lde_block<true, false, true>,
@@ -1062,12 +1129,22 @@ extern "C" int ntt_launch_lde(uint64_t *lde, const uint64_t *trace, uint64_t *sc
     {
         uint32_t log_tile = L2 + log_b;
         dim3 g((unsigned)(1u << L1), (unsigned)ncols);
-        if (L1 > NTT_MAX_STRIDED)
+        if (L1 > NTT_MAX_STRIDED) {
             hipLaunchKernelGGL((k_lde_block<false, false, true>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile),
                                st, scratch, N, lde, N << log_b, L1, L2, log_b, n_inv, rmax, tb);
-        else
+        } else if (log_b == 2 && ncols >= 2) {
+            // column pairs two lines per block; an odd last column on its own
+            dim3 g2((unsigned)(1u << L1), (unsigned)(ncols / 2));
+            hipLaunchKernelGGL(k_lde_block2, g2, dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch, N, lde,
+                               N << log_b, L1, L2, n_inv, rmax, tb);
+            if (ncols & 1)
+                hipLaunchKernelGGL((k_lde_block<false, false, false>), dim3((unsigned)(1u << L1), 1u),
+                                   dim3(1u << (log_tile - 4)), lds_bytes(log_tile), st, scratch + (ncols - 1) * N, N,
+                                   lde + (ncols - 1) * (N << log_b), N << log_b, L1, L2, log_b, n_inv, rmax, tb);
+        } else {
             hipLaunchKernelGGL((k_lde_block<false, false, false>), g, dim3(1u << (log_tile - 4)), lds_bytes(log_tile),
                                st, scratch, N, lde, N << log_b, L1, L2, log_b, n_inv, rmax, tb);
+        }
     }
     if (ev) (void)hipEventRecord(ev[2], st);
     // 3. strided forward DIT over n1
