@@ -233,8 +233,11 @@ def _leaf_roofline(width, perms, leaf_ms, leaf_bytes, traffic, launches):
     }
 
 
-LDE_KERNELS = ("k_transpose_rm2cm", "k_ntt_strided_inv<false>", "k_lde_block<false, false, false>",
-               "k_ntt_strided_fwd<false, false>")
+def _lde_kernels(nb, nbe, cols):
+    """The four kernels of one LDE as the library launches them for this shape
+    (blowup 4 with two or more columns takes the two-column block kernel)."""
+    block = "k_lde_block2" if nbe - nb == 2 and cols >= 2 else "k_lde_block<false, false, false>"
+    return ("k_transpose_rm2cm", "k_ntt_strided_inv<false>", block, "k_ntt_strided_fwd<false, false>")
 # single-rate VALU instructions of one radix-2 butterfly at the floor: a reduced
 # Goldilocks product (4 v_mad_u64_u32 + a 7-instruction reduction) plus a 6-instruction
 # add and a 6-instruction sub
@@ -259,7 +262,7 @@ def _lde_valu_roofline(nb, nbe, cols, lde_ms, config):
                     "(NExt/2 log N), %d single-rate instructions each at %.1f T lane-op/s"
                     % (INSTR_PER_BUTTERFLY, VALU_PEAK_TOPS)}
     cyc, lanes = 0.0, 0.0
-    for k in LDE_KERNELS:
+    for k in _lde_kernels(nb, nbe, cols):
         c = _profile_kernel(k, config)
         if "valu_wave_instr_per_launch" not in c:
             return out
@@ -1091,7 +1094,7 @@ def run_plain(args, cfg):
     lde_bytes = 8 * cols * (n + ne)
     pass_bytes = 16 * cols * n * 2 + 8 * cols * (n + ne) + 16 * cols * ne  # each pass reads+writes its array
     lde_traffic = None
-    tr_k = [_profile_traffic(k, args.config) for k in LDE_KERNELS]
+    tr_k = [_profile_traffic(k, args.config) for k in _lde_kernels(nb, nbe, cols)]
     if all(t is not None for t in tr_k):
         lde_traffic = sum(tr_k)
     roofline_lde = {
