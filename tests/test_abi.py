@@ -71,16 +71,19 @@ def test_set_gpu_mode_reports_gpu_backend(lib):
 
 def test_commit_host_stage_words(lib):
     """Staging size of the streamed host commit (pure host logic): three buffers of
-    2^nBits rows times the group width; one group under 256 MiB of trace, else
-    about eight groups of whole rate blocks and at least 16 columns."""
+    2^nBits rows times the widest group. One group under 256 MiB of trace; else a
+    first group of about an eighth of the rate blocks (at least 16 columns) and
+    later groups twice as wide (at most the columns left after the first); an
+    explicit width keeps every group at that width."""
     f = lib.pil2hip_commit_host_stage_words
     f.restype = ctypes.c_uint64
     f.argtypes = [ctypes.c_uint64] * 4
-    assert f(22, 128, 3, 0) == 3 * (1 << 22) * 16      # 4 GiB: eight groups of 16
+    assert f(22, 128, 3, 0) == 3 * (1 << 22) * 32      # 4 GiB: 16 columns, then 32
     assert f(16, 32, 3, 0) == 3 * (1 << 16) * 32       # 16 MiB: one group
-    assert f(20, 64, 3, 0) == 3 * (1 << 20) * 16       # 512 MiB: four groups of 16
-    assert f(22, 300, 4, 0) == 3 * (1 << 22) * 48      # rate 12: ceil(25 / 8) = 4 blocks
-    assert f(22, 128, 2, 0) == 3 * (1 << 22) * 16      # rate 4: at least 16 columns
+    assert f(20, 64, 3, 0) == 3 * (1 << 20) * 32       # 512 MiB: 16 columns, then 32
+    assert f(22, 300, 4, 0) == 3 * (1 << 22) * 96      # rate 12: 48 columns (4 blocks), then 96
+    assert f(22, 128, 2, 0) == 3 * (1 << 22) * 32      # rate 4: 16 columns, then 32
+    assert f(22, 24, 3, 0) == 3 * (1 << 22) * 16       # 768 MiB: 16, then the 8 left (under 2 * 16)
     assert f(10, 40, 3, 8) == 3 * (1 << 10) * 8        # explicit width
     assert f(10, 40, 3, 64) == 3 * (1 << 10) * 40      # clamped to the trace
     assert f(10, 0, 3, 0) == 3 * (1 << 10)             # no columns: one word per row
