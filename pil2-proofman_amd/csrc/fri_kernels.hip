@@ -11,6 +11,23 @@
 // pol[j*w + i] for w = 2^nextBits, h = 2^cur / w (elements of 3 words).
 #include "gl_field.h"
 #include "fri_kernels.h"
+#include "ntt_kernels.h"
+
+// w^-e for the roots of one split table pair (lo[e mod 2^bits] * hi[e >> bits]);
+// e is a coset index already scaled to the tables' domain. Without tables
+// (lo == nullptr), w_inv^e0 by squaring and multiplying.
+struct RootTab {
+    const uint64_t *lo, *hi;
+    uint32_t bits, shift;  // e = e0 << shift
+};
+__device__ __forceinline__ uint64_t root_pow(const RootTab &t, uint64_t w_inv, uint64_t e0)
+{
+    if (t.lo) {
+        uint64_t e = e0 << t.shift;
+        return gl_mul(t.lo[e & ((1ull << t.bits) - 1)], t.hi[e >> t.bits]);
+    }
+    return gl_pow(w_inv, e0);
+}
 
 // Fold of one group from its NX = 2^LOGX cubic values a[k][i] (component k of
 // the i-th value pol[i*2^cur + g], canonical): inverse DFT of size NX, scale
@@ -88,7 +105,7 @@ __global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, co
                                                   uint32_t cur_bits, uint64_t shift_inv, uint64_t w_inv,
                                                   uint64_t ch0, uint64_t ch1, uint64_t ch2,
                                                   const uint64_t *__restrict__ chp, uint64_t wx_inv,
-                                                  uint64_t nx_inv)
+                                                  uint64_t nx_inv, RootTab rt)
 {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t n2 = 1ull << cur_bits;
@@ -97,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fri_fold(uint64_t *__restrict__ out, co
     uint64_t a[3][1 << LOGX];
     load_group<LOGX>(a, pol, n2, g);
     // sinv_g = shift_inv * w_inv^g
-    gl3 r = fold_group<LOGX>(a, gl_mul(shift_inv, gl_pow(w_inv, g)), x, wx_inv, nx_inv);
+    gl3 r = fold_group<LOGX>(a, gl_mul(shift_inv, root_pow(rt, w_inv, g)), x, wx_inv, nx_inv);
 #pragma unroll
     for (int k = 0; k < 3; k++) out[g * 3 + k] = r.v[k];
 }
@@ -117,7 +134,8 @@ __global__ __launch_bounds__(256) void k_fri_fold_big(uint64_t *__restrict__ out
                                                       uint32_t cur_bits, uint64_t shift_inv1, uint64_t w_inv1,
                                                       uint64_t shift_inv2, uint64_t w_inv2, uint64_t ch0, uint64_t ch1,
                                                       uint64_t ch2, const uint64_t *__restrict__ chp, uint64_t w32_inv,
-                                                      uint64_t n32_inv, uint64_t wb_inv, uint64_t nb_inv)
+                                                      uint64_t n32_inv, uint64_t wb_inv, uint64_t nb_inv, RootTab rt1,
+                                                      RootTab rt2)
 {
     constexpr int NB = 1 << LOGB, GB = 256 / NB;
     __shared__ uint64_t mid[3][256];  // [k][l*GB + gl]
@@ -129,7 +147,7 @@ __global__ __launch_bounds__(256) void k_fri_fold_big(uint64_t *__restrict__ out
         uint64_t h = (uint64_t)l * n2 + g;
         uint64_t a[3][32];
         load_group<5>(a, pol, n1, h);
-        gl3 r = fold_group<5>(a, gl_mul(shift_inv1, gl_pow(w_inv1, h)), c, w32_inv, n32_inv);
+        gl3 r = fold_group<5>(a, gl_mul(shift_inv1, root_pow(rt1, w_inv1, h)), c, w32_inv, n32_inv);
 #pragma unroll
         for (int k = 0; k < 3; k++) mid[k][t] = r.v[k];
     }
@@ -145,7 +163,7 @@ __global__ __launch_bounds__(256) void k_fri_fold_big(uint64_t *__restrict__ out
 #pragma unroll
             for (int k = 0; k < 3; k++) a[k][p] = mid[k][i * GB + t];
         }
-        gl3 r = fold_group<LOGB>(a, gl_mul(shift_inv2, gl_pow(w_inv2, g)), c32, wb_inv, nb_inv);
+        gl3 r = fold_group<LOGB>(a, gl_mul(shift_inv2, root_pow(rt2, w_inv2, g)), c32, wb_inv, nb_inv);
 #pragma unroll
         for (int k = 0; k < 3; k++) out[g * 3 + k] = r.v[k];
     }
@@ -164,14 +182,23 @@ __global__ __launch_bounds__(256) void k_fri_transpose(uint64_t *__restrict__ au
 }
 
 extern "C" int fri_launch_fold(uint64_t *out, const uint64_t *pol, const uint64_t *challenge, uint64_t n_bits_ext,
-                               uint64_t prev_bits, uint64_t cur_bits, hipStream_t st)
+                               uint64_t prev_bits, uint64_t cur_bits, const NttTablesHost *tw, hipStream_t st)
 {
-    return fri_launch_fold2(out, pol, challenge, nullptr, n_bits_ext, prev_bits, cur_bits, st);
+    return fri_launch_fold2(out, pol, challenge, nullptr, n_bits_ext, prev_bits, cur_bits, tw, st);
+}
+
+// The inverse split tables of the smallest NTT domain (2^24 or 2^28) holding
+// 2^log_w points, scaled so that e0 indexes w_{2^log_w}^-e0; none above 2^28.
+static RootTab root_tab(const NttTablesHost *tw, uint64_t log_w)
+{
+    if (!tw || log_w > 28) return {nullptr, nullptr, 0, 0};
+    const NttSplitTables &S = tw->split[log_w > 24 ? 1 : 0];
+    return {S.twl_i, S.twh_i, S.bits, (uint32_t)(2 * S.bits - log_w)};
 }
 
 extern "C" int fri_launch_fold2(uint64_t *out, const uint64_t *pol, const uint64_t *challenge,
                                 const uint64_t *d_challenge, uint64_t n_bits_ext, uint64_t prev_bits,
-                                uint64_t cur_bits, hipStream_t st)
+                                uint64_t cur_bits, const NttTablesHost *tw, hipStream_t st)
 {
     if (cur_bits > prev_bits || prev_bits > n_bits_ext || prev_bits > 32) return -1;
     uint32_t logx = (uint32_t)(prev_bits - cur_bits);
@@ -205,7 +232,8 @@ extern "C" int fri_launch_fold2(uint64_t *out, const uint64_t *pol, const uint64
 #define FOLD_BIG(L)                                                                                              \
     case L:                                                                                                      \
         hipLaunchKernelGGL(k_fri_fold_big<L>, gb, dim3(256), 0, st, out, pol, (uint32_t)cur_bits, shift_inv,     \
-                           w_inv, shift_inv2, w_inv2, c0, c1, c2, d_challenge, w32_inv, n32_inv, wb_inv, nb_inv); \
+                           w_inv, shift_inv2, w_inv2, c0, c1, c2, d_challenge, w32_inv, n32_inv, wb_inv, nb_inv,  \
+                           root_tab(tw, prev_bits), root_tab(tw, prev_bits - 5));                                \
         break;
         switch (logb) {
             FOLD_BIG(1)
@@ -224,7 +252,7 @@ extern "C" int fri_launch_fold2(uint64_t *out, const uint64_t *pol, const uint64
 #define FOLD_CASE(L)                                                                                        \
     case L:                                                                                                 \
         hipLaunchKernelGGL(k_fri_fold<L>, g, dim3(256), 0, st, out, pol, (uint32_t)cur_bits, shift_inv,     \
-                           w_inv, c0, c1, c2, d_challenge, wx_inv, nx_inv);                                 \
+                           w_inv, c0, c1, c2, d_challenge, wx_inv, nx_inv, root_tab(tw, prev_bits));        \
         break;
     switch (logx) {
         FOLD_CASE(1)

@@ -129,6 +129,25 @@ static int64_t ensure_tables(NttTablesHost **out)
     return 0;
 }
 
+// The root tables for the FRI folds. Setting them up copies them to the device
+// synchronously, which a stream capture does not allow: on a stream that is not
+// being captured they are set up here if needed; under capture only tables set
+// up earlier are used, and without them the folds raise the root to a power.
+static const NttTablesHost *fri_tables(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        cs = hipStreamCaptureStatusActive;
+    }
+    NttTablesHost *tb = nullptr;
+    if (cs == hipStreamCaptureStatusNone) return ensure_tables(&tb) ? nullptr : tb;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_tables_ready[dev] ? &g_tables[dev] : nullptr;
+}
+
 // ----------------------------------------------------------- phase timing --
 // When enabled, pil2hip_commit records HIP events on the caller's stream
 // between its kernels and accumulates the elapsed time per phase:
@@ -1246,7 +1265,8 @@ int64_t pil2hip_fri_fold(uint64_t *d_out, const uint64_t *d_pol, const uint64_t 
 {
     if (step == 0) return 0;
     if (!challenge) return fail(-13, "null challenge");
-    int e = fri_launch_fold(d_out, d_pol, challenge, nBitsExt, prevBits, currentBits, (hipStream_t)stream);
+    int e = fri_launch_fold(d_out, d_pol, challenge, nBitsExt, prevBits, currentBits, fri_tables((hipStream_t)stream),
+                            (hipStream_t)stream);
     if (e == -1) return fail(-16, "invalid fold bits");
     if (e == -2) return fail(-17, "fold ratio must be 2^1 .. 2^10");
     if (e) return fail(-100 - e, "fold launch failed");
@@ -1317,7 +1337,7 @@ static int64_t fri_enqueue(const uint64_t *d_pol, uint64_t *d_work, const uint64
     for (uint64_t s = 0; s < n && !rc; s++) {
         if (s > 0) {
             int e = fri_launch_fold2(d_work, s == 1 ? d_pol : d_work, nullptr, d_ch + 3 * s, nbe, steps[s - 1],
-                                     steps[s], st);
+                                     steps[s], fri_tables(st), st);
             if (e) rc = fail(-100 - e, "fold launch failed");
         }
         if (!rc && s + 1 < n) {
@@ -1367,6 +1387,7 @@ int64_t pil2hip_fri_layers(const uint64_t *d_pol, uint64_t *d_work, const uint64
     for (uint64_t s = 0; s + 1 < nSteps; s++)
         if (!d_sources[s] || !d_nodes[s]) return fail(-13, "null layer buffer");
     hipStream_t st = (hipStream_t)stream;
+    (void)fri_tables(st);  // set up before the capture below, which only looks them up
     if (!graph)
         return fri_enqueue(d_pol, d_work, d_challenges, nBitsExt, steps, nSteps, arity, d_sources, d_nodes, st, false);
     std::vector<uint64_t> key = {(uint64_t)(uintptr_t)d_pol, (uint64_t)(uintptr_t)d_work,
