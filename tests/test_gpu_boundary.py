@@ -156,6 +156,22 @@ def test_fri_fold_large_ratios_match_oracle(pc, oracle, prev, cur, nbe):
     assert np.array_equal(pc.to_host(d)[:3 << cur], want)
 
 
+@pytest.mark.parametrize("prev,cur", [(25, 22), (25, 19)])
+def test_fri_fold_above_2_24_matches_oracle(pc, oracle, prev, cur):
+    """Folds of a 2^25-point layer: the lanes' coset roots w_{2^25}^-g come from the
+    2^28 domain's split root tables (the 2^24 ones cover layers up to 2^24 points);
+    ratio 64 also takes the second fold's roots from the 2^24 tables."""
+    pol = datagen.uniform(3 << prev, 0x5EED0300 + prev * 16 + cur)
+    ch = datagen.uniform(3, 0x5EED0400 + cur)
+    want = oracle.fri_fold(pol, ch, 1, prev, prev, cur)
+    print("fri fold", prev, cur, flush=True)
+    d = pc.to_device(pol)
+    out = pc.empty(3 << cur)
+    pc.FRI.fold(1, d, ch, prev, prev, cur, out=out)
+    _sync()
+    assert np.array_equal(pc.to_host(out), want)
+
+
 def test_fri_fold_ratio_limit(pc):
     d = pc.empty(3 << 12)
     ch = np.array([1, 2, 3], dtype=np.uint64)
