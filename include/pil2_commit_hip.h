@@ -246,7 +246,7 @@ int64_t pil2hip_commit_batch(const Pil2CommitJob *jobs, uint64_t n_jobs, uint32_
  *                     rounds of at most 4 bits
  *   coop_max_parents  tree levels with at most this many parents hash with 16 lanes
  *                     per permutation, larger levels with one lane per parent
- * The plain entry points and fields left 0 use {-, 13, 3, 16384}. */
+ * The plain entry points and fields left 0 use {-, 13, 4, 16384}. */
 typedef struct Pil2CommitPlan {
     uint32_t n_streams, ntt_tile_log, ntt_radix_log, coop_max_parents;
 } Pil2CommitPlan;

@@ -186,8 +186,10 @@ static int64_t check_arity(uint64_t arity)
     return 0;
 }
 
-// measured on the plain and mixed-batch workloads: 2^13 tiles (two blocks per CU) and radix-8 rounds
-static const Pil2CommitPlan kDefaultPlan = {1, 13, 3, P2_COOP_MAX_PARENTS};
+// measured on the plain workloads: 2^13 strided tiles (two blocks per CU) and radix-16
+// block rounds (plain-large 118.9 ms against 119.4 ms with radix-8 rounds, now that blowup
+// 4 runs two columns per block; plain-small the same either way)
+static const Pil2CommitPlan kDefaultPlan = {1, 13, 4, P2_COOP_MAX_PARENTS};
 
 static int64_t lde_impl(uint64_t *d_lde, const uint64_t *d_trace, uint64_t *d_scratch, uint64_t nBits,
                         uint64_t nBitsExt, uint64_t nCols, hipStream_t st, hipEvent_t *ev,
@@ -601,7 +603,7 @@ int64_t pil2hip_compute_q(uint64_t *d_cmQ, uint64_t *d_nodes, const uint64_t *d_
 // batch after one warm replay). A plan with every field 0 also takes each job's
 // NTT tile, radix and level threshold from the shape wisdom below (measured on
 // first sight of a shape); otherwise plan fields left 0 take their defaults
-// (tile 2^13, radix 8, cooperative levels up to 16,384 parents).
+// (tile 2^13, radix 16, cooperative levels up to 16,384 parents).
 struct BatchGraph {
     std::vector<uint64_t> key;
     Pil2CommitPlan plan;

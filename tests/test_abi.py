@@ -101,7 +101,7 @@ def test_batch_shape_plan_defaults_before_tuning(lib):
     f.argtypes = [ctypes.c_uint64] * 4 + [ctypes.c_void_p]
     p = Plan()
     assert f(20, 22, 100, 3, ctypes.byref(p)) == 0
-    assert (p.n_streams, p.ntt_tile_log, p.ntt_radix_log, p.coop_max_parents) == (0, 13, 3, 16384)
+    assert (p.n_streams, p.ntt_tile_log, p.ntt_radix_log, p.coop_max_parents) == (0, 13, 4, 16384)
     assert f(20, 22, 100, 3, None) < 0
 
 

@@ -208,7 +208,7 @@ def test_batch_plan_defaults_and_errors(pc):
     b = pc.CommitBatch(jobs, arity=3)
     assert b.launch(plan=dict(n_streams=2)) == 2
     _sync()
-    assert b.plan == dict(n_streams=2, ntt_tile_log=13, ntt_radix_log=3, coop_max_parents=16384)
+    assert b.plan == dict(n_streams=2, ntt_tile_log=13, ntt_radix_log=4, coop_max_parents=16384)
     for k, j in enumerate(b.jobs):
         assert np.array_equal(pc.to_host(j["nodes"]), want[k][1]), k
     for bad in (pc.Pil2CommitPlan(2, 15, 0, 0), pc.Pil2CommitPlan(2, 11, 0, 0), pc.Pil2CommitPlan(2, 0, 5, 0),
