@@ -73,10 +73,10 @@ __global__ __launch_bounds__(256) void k_transpose_rm2cm(const uint64_t *__restr
     uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (uint32_t k = ty; k < TR_T; k += 8) {
         uint64_t r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[k][tx] = in[r * in_rs + c];
+        if (r < rows && c < cols) tile[k][tx] = __builtin_nontemporal_load(&in[r * in_rs + c]);
     }
     __syncthreads();
-    for (uint32_t k = ty; k < TR_T; k += 8) {  // streaming stores (the pass's output is not re-read soon)
+    for (uint32_t k = ty; k < TR_T; k += 8) {  // loads and stores are non-temporal: each word moves once
         uint64_t c = c0 + k, r = r0 + tx;
         if (r < rows && c < cols) __builtin_nontemporal_store(tile[tx][k], &out[c * out_cs + r]);
     }
@@ -342,8 +342,9 @@ __device__ __forceinline__ uint32_t strided_tile(uint32_t bx, uint32_t lt)
 // before the first global store), so a wave keeps sixteen HBM requests in flight
 // instead of waiting out one round trip per element.
 // CANON: the words may be non-canonical (a pass over caller data); the forward
-// pass reads the block pass's canonical output and loads it as is.
-template <bool CANON>
+// pass reads the block pass's canonical output and loads it as is. NT: non-temporal
+// loads (measured faster for the inverse pass, slower for the forward pass).
+template <bool CANON, bool NT>
 __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uint64_t *__restrict__ col, uint64_t n2,
                                              uint64_t i0, uint32_t lt)
 {
@@ -351,7 +352,8 @@ __device__ __forceinline__ void strided_load(uint64_t *__restrict__ s, const uin
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         uint32_t x = threadIdx.x + k * blockDim.x;
-        v[k] = col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))];
+        const uint64_t *a = &col[(uint64_t)(x >> lt) * n2 + i0 + (x & ((1u << lt) - 1))];
+        v[k] = NT ? __builtin_nontemporal_load(a) : *a;
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) s[lds_addr(threadIdx.x + k * blockDim.x)] = CANON ? gl_canon(v[k]) : v[k];
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_inv(uint64_t *__res
     extern __shared__ uint64_t s[];
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
-    strided_load<true>(s, col, B.stride, 0, lt);
+    strided_load<true, true>(s, col, B.stride, 0, lt);
     // the fewest rounds of at most 4 bits, as even as possible (10 bits: 4 + 3 + 3); the
     // strided tile keeps 16 values per thread whatever the plan's radix
     int q = (int)L, nr = ((int)L + 3) / 4, kr = 0;
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_strided_fwd(uint64_t *__res
     extern __shared__ uint64_t s[];
     StridedBlock B = strided_block<SPLIT>(lt, log_n2, qlo, L);
     uint64_t *col = data + (uint64_t)blockIdx.y * cs + B.base;
-    strided_load<false>(s, col, B.stride, 0, lt);
+    strided_load<false, false>(s, col, B.stride, 0, lt);
     uint32_t q = 0, nr = (L + 3) / 4, kr = 0;  // rounds as in k_ntt_strided_inv
     while (q < L) {
         uint32_t r = ((L - q) + (nr - kr) - 1) / (nr - kr);
